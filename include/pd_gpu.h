@@ -1,0 +1,205 @@
+/*
+ * pd_gpu.h -- C-ABI of the MI355X peridynamic time-step engine (gfx950).
+ *
+ * Drop-in boundary for the hot path of XiaoMmingW/PD_Gpu1: the stages its
+ * solvers call (Coord -> Neighbor -> Surf_Corr -> Initial -> Force ->
+ * Integrate/Reduce -> Save).  Every entry point below names the reference
+ * function or kernel it replaces (file:line in the reference tree).
+ *
+ * Conventions
+ *  - All functions are extern "C", take an opaque pd_handle* and return an int
+ *    status (PD_OK = 0).  pd_last_error() returns the message of the last
+ *    failure on that handle.  The reference's CHECK() printed and exit(1)ed
+ *    (Error.cuh:9-23); nothing here exits the process.
+ *  - The handle owns all device memory and one HIP stream.  Host buffers are
+ *    caller-owned: inputs are copied in, outputs are written to caller
+ *    buffers of the documented size, no pointer is retained.
+ *  - A handle is single-threaded; distinct handles are independent.  There are
+ *    no globals and no __constant__ state (the reference keeps one problem per
+ *    process in `dp`/`p`, main.cu:6-9).
+ *  - Per-particle vector fields are SoA doubles laid out [dim][N].
+ *  - Bonds are CSR: row_ptr[N+1] and col[Nb] (int32), each row ascending (the
+ *    slot order of the reference's CPU neighbour search, Neighbor.cu:44-58).
+ *  - Fail state is a bit mask: bit (k & 31) of word k >> 5 is 1 while directed
+ *    bond k is intact (the reference's int fail[] slot, Initial.cu:6-14).
+ */
+#ifndef PD_GPU_H
+#define PD_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PD_OK 0
+#define PD_E_INVALID 1     /* bad argument or call order */
+#define PD_E_HIP 2         /* HIP runtime error */
+#define PD_E_CAPACITY 3    /* neighbour count above max_neighbors (MN) or size limit */
+#define PD_E_NONFINITE 4   /* NaN/Inf met in a reduction */
+#define PD_E_NOT_CONVERGED 5
+
+/* models (Dselect::B_S, Base.cuh:40) */
+#define PD_MODEL_PMB 1
+#define PD_MODEL_LPS 2
+
+/* arithmetic of the force kernels */
+#define PD_PREC_FP64 0     /* FP64 everywhere, bonds summed in CSR order */
+#define PD_PREC_MIXED 1    /* FP32 bond math with compensated sums; FP64 state,
+                              FP64 re-evaluation of near-critical bonds */
+
+/* fields for pd_get_field / pd_set_field */
+#define PD_FIELD_X0 0      /* reference coordinates [dim][N] */
+#define PD_FIELD_U 1       /* displacement [dim][N] */
+#define PD_FIELD_V 2       /* velocity [dim][N] */
+#define PD_FIELD_A 3       /* acceleration [dim][N] */
+#define PD_FIELD_F 4       /* pair-force density [dim][N] (FP32 in mixed mode, returned widened) */
+#define PD_FIELD_B 5       /* body force [dim][N] */
+#define PD_FIELD_VH 6      /* ADR half-step velocity [dim][N] */
+#define PD_FIELD_VHO 7     /* ADR previous half-step velocity [dim][N] */
+#define PD_FIELD_FO 8      /* ADR previous force [dim][N] */
+#define PD_FIELD_DMG 9     /* damage [N] */
+#define PD_FIELD_THETA 10  /* LPS dilatation [N] */
+#define PD_FIELD_M 11      /* LPS weighted volume [N] */
+#define PD_FIELD_IDIST 20  /* bond reference length xi [Nb] */
+#define PD_FIELD_FAC 21    /* partial-volume factor [Nb] */
+#define PD_FIELD_SCR 22    /* surface-correction factor [Nb] */
+#define PD_FIELD_W 23      /* LPS influence weight [Nb] */
+#define PD_FIELD_E 24      /* LPS extension of the last theta pass [Nb] (FP64 mode) */
+
+/* init modes for pd_init_state */
+#define PD_INIT_DYNAMIC 0  /* base_integrate_initial_* (Initial.cu:24-55,82-112) */
+#define PD_INIT_STATIC 1   /* static_initial_bond_* (Initial.cu:57-75,114-132) */
+
+/* Model parameters: the device_parameter the reference copies to __constant__
+ * dp (Base.cuh:46-68) plus the switches its code hard-codes. */
+typedef struct pd_params {
+    int32_t dim;               /* 2 or 3 (Dselect::Dim) */
+    int32_t model;             /* PD_MODEL_PMB or PD_MODEL_LPS */
+    int32_t precision;         /* PD_PREC_FP64 or PD_PREC_MIXED */
+    int32_t max_neighbors;     /* MN: 32 in 2D, 128 in 3D (Base.cuh:27-29) */
+    int32_t theta_skip_broken; /* 1: dilatation skips broken bonds (Force_State.cu:345) */
+    int32_t reduce_lanes;      /* power-of-two lane count T of the fixed reduction tree */
+    int32_t n_top, n_bot;      /* strip particles stored after N_int (Coord.cu:47-66) */
+    double dx, delta, vol, bc, mass, K, G;
+    double emod, pratio, dens, s0, dt;
+    double break_y_min;        /* 3D breaks need y_i, y_j >= this (Force_Bond.cu:117: -0.04) */
+    double strip_velocity;     /* strip load speed (Solve_Plate_Crack_Grow.cu:128: 20 m/s) */
+    double near_window;        /* mixed mode: |s-s0| <= near_window*s0 is re-evaluated in FP64 */
+} pd_params;
+
+/* Cylinder impactor (Cylinder_Parameter, Base.cuh:227-236). */
+typedef struct pd_cyl {
+    double rad, mass, height, v, x, y, z;
+} pd_cyl;
+
+typedef struct pd_handle pd_handle;
+
+/* ---- parameters: BaseModel_Parameter::cal_parameter (Base.cuh:114-183) ----
+ * Derives delta, nx/ny/nz, bc, vol, mass, K, G exactly as the reference does.
+ * delta_factor is 3.015 in the reference (Base.cuh:116).  ref_k_bug = 1 keeps
+ * the 3D LPS K = E/3/(1-2G) of Base.cuh:175; 0 uses K = E/(3(1-2 nu)).
+ * Writes lattice counts to nxyz[3] and the particle count nx*ny*nz to *n. */
+int pd_params_init(pd_params *p, int dim, int model, double dt, double dx, double thick,
+                   double length, double height, double width, double emod,
+                   double pratio, double dens, double s0, double delta_factor,
+                   int ref_k_bug, int *nxyz, int *n);
+
+/* ---- coordinate generators (host, bit-identical to the reference) -------- */
+int pd_gen_plate(int nx, int ny, double dx, double *x, double *y);          /* Coord.cu:19-31 */
+int pd_gen_plate_strips(int nx, int ny, double dx, double *x, double *y);   /* Coord.cu:47-66 */
+/* Coord.cu:77-129.  Returns the particle count; x/y/z may be NULL to count. */
+int pd_gen_kw(int nx, int ny, int nz, double dx, double rad, double *x, double *y, double *z);
+/* Jittered cuboid lattice, SplitMix64 stream (no reference counterpart). */
+int pd_gen_block_jitter(int nx, int ny, int nz, double dx, double jitter, uint64_t seed,
+                        double *x, double *y, double *z);
+
+/* ---- memory: Base_Allocate / Base_Free (Memory.cu:139-147,276-284) ------- */
+int pd_create(const pd_params *p, int device, pd_handle **out);
+int pd_destroy(pd_handle *h);
+const char *pd_last_error(pd_handle *h);
+int pd_get_params(pd_handle *h, pd_params *out);
+int pd_set_params(pd_handle *h, const pd_params *p); /* scalars only (dt, s0, ...) */
+
+/* ---- coordinates: H2D copy of Solve_Kalthoff_Winkler.cu:381-383 ---------- */
+int pd_set_positions(pd_handle *h, const double *x, const double *y, const double *z,
+                     int n_int, int n);
+
+/* ---- neighbours: kernel_find_neighbor_2D/3D (Neighbor.cu:7-40) ----------- */
+int pd_build_neighbors(pd_handle *h);            /* cell list -> sorted CSR */
+int pd_num_bonds(pd_handle *h, int64_t *nb);
+int pd_get_csr(pd_handle *h, int32_t *row_ptr, int32_t *col);
+
+/* ---- volume and surface correction (Surf_Corr.cu:7-35, 235-310) --------- */
+int pd_vol_corr(pd_handle *h);
+int pd_surface_correct(pd_handle *h);            /* sets scr; scr = 1 until called */
+
+/* ---- initial state (Initial.cu:6-75) ------------------------------------- */
+int pd_init_state(pd_handle *h, int mode);
+
+/* ---- cracks: kernel_set_crack_2D (Base_Function.cu:39-81),
+ *      kernel_set_crack (Solve_Kalthoff_Winkler.cu:40-63) ------------------ */
+int pd_set_crack_2d(pd_handle *h, double clength, double loc_x, double loc_y, double theta);
+int pd_set_notches_kw(pd_handle *h, double rad);
+
+/* ---- LPS weights: kernel_initial_weight_2D/3D (Force_State.cu:8-54) ------ */
+int pd_lps_weights(pd_handle *h);
+
+/* ---- force: kernel_particle_force_2D / kernel_bond_force_3D
+ *      (Force_Bond.cu:52-141); kernel_cal_theta_* + kernel_state_force_*
+ *      (Force_State.cu:56-306).  PMB rows i < N; LPS theta i < N, force
+ *      i < N_int.  Bonds with s > s0 are marked broken after their force is
+ *      added (Force_Bond.cu:32). ------------------------------------------- */
+int pd_force(pd_handle *h);
+
+/* ---- explicit integration: kernel_integrate_{2D,3D}_{1,2}
+ *      (Integrate.cu:174-249) and kernel_quick_load
+ *      (Solve_Plate_Crack_Grow.cu:8-19, applied inside half 1) ------------- */
+int pd_verlet_half1(pd_handle *h);
+int pd_verlet_half2(pd_handle *h);
+/* One full explicit step as the reference time loops order it
+ * (Solve_Plate_Crack_Grow.cu:126-149, Solve_Kalthoff_Winkler.cu:407-430),
+ * repeated nsteps times from a captured HIP graph. */
+int pd_run_verlet(pd_handle *h, int nsteps);
+
+/* ---- ADR: kernel_cnn + cal_cn + sum + kernel_integrate
+ *      (Integrate.cu:7-129, Reduce.cu:11-65) ------------------------------- */
+int pd_set_body_force_tensile(pd_handle *h, int nx, int ny); /* Solve_Plate_Tensile.cu:7-17 */
+/* One ADR iteration: force, fused damping/residual reduction on the device,
+ * update.  c_out / resid_out may be NULL (then no host synchronisation). */
+int pd_adr_iterate(pd_handle *h, int ct, double *c_out, double *resid_out);
+/* Iterate from ct0 until the residual <= tol or max_iters; device-side stop. */
+int pd_adr_solve(pd_handle *h, int ct0, int max_iters, double tol,
+                 int *iters_out, double *resid_out);
+/* Strip displacement increment for displacement-controlled loading. */
+int pd_strip_displace(pd_handle *h, double du_top, double du_bot);
+/* Reduce.cu:52-65 sum() test hook: fixed-tree sum of field component comp. */
+int pd_reduce_sum(pd_handle *h, int field, int comp, double *out);
+
+/* ---- damage and fields (Base_Function.cu:149-217) ------------------------ */
+int pd_damage(pd_handle *h);
+int pd_get_field(pd_handle *h, int field, double *out);
+int pd_set_field(pd_handle *h, int field, const double *in);
+int pd_get_fail_bits(pd_handle *h, uint32_t *out);   /* (Nb+31)/32 words */
+int pd_set_fail_bits(pd_handle *h, const uint32_t *in);
+
+/* ---- KW cylinder and contact (Solve_Kalthoff_Winkler.cu:9-37,89-265) ---- */
+int pd_cylinder_init(pd_handle *h, const pd_cyl *c);
+int pd_cylinder_get(pd_handle *h, double *state12); /* vel[3] disp[3] acc[3] force[3] */
+int pd_contact_step(pd_handle *h);  /* cylinder half 1 is in pd_verlet_half1 */
+
+/* ---- output (Save.cu:6-140) --------------------------------------------- */
+#define PD_LAYOUT_PLATE 0
+#define PD_LAYOUT_KW 1
+int pd_save_text(pd_handle *h, const char *path, int layout, int nx, int ny, int nz, double rad);
+int pd_checkpoint_write(pd_handle *h, const char *path, int64_t step);
+int pd_checkpoint_read(pd_handle *h, const char *path, int64_t *step);
+
+/* ---- timing: device-event time of the last force launch(es) ------------- */
+int pd_last_force_ms(pd_handle *h, double *ms);
+int pd_sync(pd_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,389 @@
+// k_integrate.hip -- velocity Verlet, adaptive dynamic relaxation (ADR) with
+// a fused on-device damping reduction, the Kalthoff-Winkler cylinder contact,
+// and the fixed-order reduction they share.
+//
+// Fixed reduction order (the "canonical tree", lane count T = blocks*threads,
+// a power of two): lane t sums elements t, t+T, t+2T, ... in sequence; the T
+// partials are then combined pairwise at offsets 1, 2, 4, ... (partial[t] +=
+// partial[t+off] for t % (2*off) == 0) -- xor/down shuffles inside a wave64,
+// LDS across waves, a last-block pass across blocks.  The result is
+// deterministic and independent of scheduling; the FP64 oracle emulates the
+// same order, so ADR damping coefficients and trajectories match it bit for
+// bit.  This replaces reduce_cp + sum() (Reduce.cu:11-65), which copies each
+// sum to the host with a blocking cudaMemcpy, twice per ADR iteration
+// (Integrate.cu:112-113).
+#include "pd_internal.h"
+
+namespace {
+
+template <int NC>
+struct Vec {
+    double v[NC];
+};
+
+// Combine per-lane partials of one block in canonical order; the block
+// partial lands in out[] of thread 0.  B = blockDim.x (power of two, >= 64).
+template <int NC>
+__device__ __forceinline__ void block_canon(Vec<NC> &p, double (*sh)[16])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            double o = __shfl_down(p.v[c], off, 64);
+            p.v[c] = p.v[c] + o;
+        }
+    }
+    if (lane == 0)
+        for (int c = 0; c < NC; ++c) sh[c][wave] = p.v[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int off = 1; off < nwaves; off <<= 1)
+            for (int w = 0; w < nwaves; w += 2 * off)
+                for (int c = 0; c < NC; ++c) sh[c][w] = sh[c][w] + sh[c][w + off];
+        for (int c = 0; c < NC; ++c) p.v[c] = sh[c][0];
+    }
+}
+
+// Cross-block finish: returns true in thread 0 of the last block, with the
+// grand totals in tot[].
+template <int NC>
+__device__ __forceinline__ bool grid_canon(Vec<NC> &p, double *partials, DevScalars *S, double (*sh)[16],
+                                           Vec<NC> &tot)
+{
+    __shared__ bool last;
+    block_canon<NC>(p, sh);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < NC; ++c) partials[blockIdx.x * NC + c] = p.v[c];
+        __threadfence();
+        unsigned t = atomicAdd(&S->ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return false;
+    __threadfence();
+    const int G = gridDim.x;
+    // partials are read through volatile loads so the last block sees the
+    // values published before each block's ticket.
+    volatile double *vp = partials;
+    for (int off = 1; off < G; off <<= 1)
+        for (int b = 0; b < G; b += 2 * off)
+            for (int c = 0; c < NC; ++c) vp[b * NC + c] = vp[b * NC + c] + vp[(b + off) * NC + c];
+    for (int c = 0; c < NC; ++c) tot.v[c] = vp[c];
+    S->ticket = 0u;
+    return true;
+}
+
+__global__ void k_verlet1(int dim, int N_int, int n_top, int n_bot, double dt, double load, D3 v, CD3 a,
+                          D3 u, double *cyl)
+{
+    // Integrate.cu:447-463 (i < N_int) and Solve_Plate_Crack_Grow.cu:20-27
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N_int) {
+        for (int d = 0; d < dim; ++d) {
+            v.p[d][i] += (dt / 2.0) * a.p[d][i];
+            u.p[d][i] += v.p[d][i] * dt;
+        }
+    } else if (i < N_int + n_top) {
+        u.p[1][i] += load * dt;
+    } else if (i < N_int + n_top + n_bot) {
+        u.p[1][i] -= load * dt;
+    }
+    if (cyl && i == 0) {
+        // Solve_Kalthoff_Winkler.cu:101-110: vel += dt/2*acc; disp += vel*dt
+        for (int d = 0; d < 3; ++d) {
+            cyl[d] += (dt / 2.0) * cyl[6 + d];
+            cyl[3 + d] += cyl[d] * dt;
+        }
+    }
+}
+
+__global__ void k_verlet2(int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f, CF3 f32, int use32,
+                          CD3 b)
+{
+    // Integrate.cu:465-481
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N_int) return;
+    for (int d = 0; d < dim; ++d) {
+        double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
+        double ad = (fd + b.p[d][i]) / dens;
+        a.p[d][i] = ad;
+        v.p[d][i] += (dt / 2.0) * ad;
+    }
+}
+
+__device__ __forceinline__ double adr_c(double cn1, double cn2)
+{
+    // Integrate.cu:117-128
+    double cn;
+    if (cn2 != 0.0) {
+        if (cn1 / cn2 > 0.0)
+            cn = 2.0 * sqrt(cn1 / cn2);
+        else
+            cn = 0.0;
+    } else {
+        cn = 0.0;
+    }
+    if (cn > 2.0) cn = 1.9;
+    return cn;
+}
+
+__global__ __launch_bounds__(1024) void k_adr_reduce(int dim, int N_int, int N, double mass, double dt, CD3 u,
+                                                     CD3 f, CF3 f32, int use32, CD3 fo, CD3 vho, CD3 b,
+                                                     double *partials, DevScalars *S, double tol)
+{
+    __shared__ double sh[4][16];
+    if (S->done) return;
+    const int T = gridDim.x * blockDim.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    Vec<4> p{{0.0, 0.0, 0.0, 0.0}};
+    for (int i = t; i < N; i += T) {
+        double cn = 0.0, disp = 0.0, rn = 0.0, rd = 0.0;
+        if (i < N_int) {
+            // kernel_cnn, Integrate.cu:19-31
+            for (int d = 0; d < dim; ++d) {
+                double ud = u.p[d][i];
+                double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
+                double vd = vho.p[d][i];
+                if (d == 0) {
+                    if (vd != 0.0) cn = -1.0 * ud * ud * (fd / mass - fo.p[d][i] / mass) / (dt * vd);
+                    disp = ud * ud;
+                } else {
+                    if (vd != 0.0) cn -= 1.0 * ud * ud * (fd / mass - fo.p[d][i] / mass) / (dt * vd);
+                    disp += ud * ud;
+                }
+                double r = fd + b.p[d][i];
+                rn += r * r;
+                rd += b.p[d][i] * b.p[d][i];
+            }
+        } else {
+            for (int d = 0; d < dim; ++d) {
+                double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
+                rd += fd * fd;
+            }
+        }
+        p.v[0] += cn;
+        p.v[1] += disp;
+        p.v[2] += rn;
+        p.v[3] += rd;
+    }
+    Vec<4> tot;
+    if (grid_canon<4>(p, partials, S, sh, tot)) {
+        double c = adr_c(tot.v[0], tot.v[1]);
+        double rnum = tot.v[2], rden = tot.v[3];
+        double r = rden > 0.0 ? sqrt(rnum) / sqrt(rden) : (rnum > 0.0 ? INFINITY : 0.0);
+        S->c = c;
+        S->resid = r;
+        for (int k = 0; k < 4; ++k) S->sums[k] = tot.v[k];
+        if (!(isfinite(tot.v[0]) && isfinite(tot.v[1]) && isfinite(rnum) && isfinite(rden))) S->nonfinite = 1;
+        S->ct_use = S->ct_counter;
+        S->ct_counter = S->ct_counter + 1;
+        if (tol >= 0.0 && r <= tol) S->done = 1;
+    }
+}
+
+__global__ void k_adr_update(int dim, int N_int, double mass, double dt, CD3 f, CF3 f32, int use32, CD3 b,
+                             D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S)
+{
+    // kernel_integrate, Integrate.cu:62-105
+    if (S->done) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N_int) return;
+    const int ct = S->ct_use;
+    const double c = S->c;
+    for (int d = 0; d < dim; ++d) {
+        double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
+        double hv;
+        if (ct == 0)
+            hv = 1.0 * dt / mass * (fd + b.p[d][i]) / 2.0;
+        else
+            hv = ((2.0 - c * dt) * vho.p[d][i] + 2.0 * dt / mass * (fd + b.p[d][i])) / (2.0 + c * dt);
+        vh.p[d][i] = hv;
+        v.p[d][i] = 0.5 * (vho.p[d][i] + hv);
+        u.p[d][i] += hv * dt;
+        vho.p[d][i] = hv;
+        fo.p[d][i] = fd;
+    }
+}
+
+__global__ void k_set_ct(DevScalars *S, int ct, int clear_done)
+{
+    S->ct_counter = ct;
+    if (clear_done) {
+        S->done = 0;
+        S->iters = 0;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_contact(int N, const double *__restrict__ dmg, CD3 x, D3 u, D3 v,
+                                                  pd_cyl cyl, double dt, double dens, double vol,
+                                                  double *partials, DevScalars *S)
+{
+    // Solve_Kalthoff_Winkler.cu:177-239; the reaction is summed in canonical
+    // order instead of by float atomicAdd (Solve_Kalthoff_Winkler.cu:168-170),
+    // then the cylinder's second half step (Solve_Kalthoff_Winkler.cu:255-265)
+    // runs in the last block.
+    __shared__ double sh[3][16];
+    const int T = gridDim.x * blockDim.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const double *cd = S->cyl + 3;
+    const double cdx = cd[0], cdy = cd[1], cdz = cd[2];
+    Vec<3> p{{0.0, 0.0, 0.0}};
+    for (int i = t; i < N; i += T) {
+        double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+        if (dmg[i] < 1.0) {
+            double dpx = x.p[0][i] + u.p[0][i] - cyl.x - cdx;
+            double dpy = x.p[1][i] + u.p[1][i] - cyl.y - cdy;
+            double dpz = x.p[2][i] + u.p[2][i] - cyl.z - cdz;
+            double dpd = sqrt(dpx * dpx + dpz * dpz);
+            if (dpd <= cyl.rad && fabs(dpy) <= (cyl.height / 2.0)) {
+                double rx, ry, rz;
+                if (cyl.rad - dpd <= cyl.height / 2.0 - fabs(dpy)) {
+                    rx = dpx / dpd * cyl.rad;
+                    ry = dpy;
+                    rz = dpz / dpd * cyl.rad;
+                } else if (dpy >= 0.0) {
+                    rx = dpx;
+                    ry = cyl.height / 2.0;
+                    rz = dpz;
+                } else {
+                    rx = dpx;
+                    ry = -cyl.height / 2.0;
+                    rz = dpz;
+                }
+                double oux = u.p[0][i], ouy = u.p[1][i], ouz = u.p[2][i];
+                double ovx = v.p[0][i], ovy = v.p[1][i], ovz = v.p[2][i];
+                double nux = cyl.x + cdx + rx - x.p[0][i];
+                double nuy = cyl.y + cdy + ry - x.p[1][i];
+                double nuz = cyl.z + cdz + rz - x.p[2][i];
+                u.p[0][i] = nux;
+                u.p[1][i] = nuy;
+                u.p[2][i] = nuz;
+                double nvx = (nux - oux) / dt, nvy = (nuy - ouy) / dt, nvz = (nuz - ouz) / dt;
+                v.p[0][i] = nvx;
+                v.p[1][i] = nvy;
+                v.p[2][i] = nvz;
+                r0 = -dens * (nvx - ovx) / dt * vol;
+                r1 = -dens * (nvy - ovy) / dt * vol;
+                r2 = -dens * (nvz - ovz) / dt * vol;
+            }
+        }
+        p.v[0] += r0;
+        p.v[1] += r1;
+        p.v[2] += r2;
+    }
+    Vec<3> tot;
+    if (grid_canon<3>(p, partials, S, sh, tot)) {
+        double *cs = S->cyl;  // vel[0..2] disp[3..5] acc[6..8] force[9..11]
+        for (int d = 0; d < 3; ++d) {
+            cs[9 + d] = tot.v[d];
+            S->sums[4 + d] = tot.v[d];
+            cs[6 + d] = cs[9 + d] / cyl.mass;
+            cs[d] += dt / 2.0 * cs[6 + d];
+            cs[9 + d] = 0.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_field_sum(const double *__restrict__ v, int64_t n, double *partials,
+                                                    DevScalars *S)
+{
+    __shared__ double sh[1][16];
+    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Vec<1> p{{0.0}};
+    for (int64_t i = t; i < n; i += T) p.v[0] += v[i];
+    Vec<1> tot;
+    if (grid_canon<1>(p, partials, S, sh, tot)) S->sums[7] = tot.v[0];
+}
+
+__global__ void k_strip_displace(int N_int, int n_top, int n_bot, double dtop, double dbot, double *uy)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_top)
+        uy[N_int + k] += dtop;
+    else if (k < n_top + n_bot)
+        uy[N_int + k] += dbot;
+}
+
+__global__ void k_f32_to_f64(const float *a, double *b, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+} // namespace
+
+hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
+                          D3 v, CD3 a, D3 u, double *cyl_state)
+{
+    int n = N_int + n_top + n_bot;
+    if (n < 1) n = 1;
+    k_verlet1<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, load, v, a, u, cyl_state);
+    return hipGetLastError();
+}
+
+hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f,
+                          CF3 f32, int use32, CD3 b)
+{
+    if (N_int <= 0) return hipSuccess;
+    k_verlet2<<<nblk(N_int, 256), 256, 0, s>>>(dim, N_int, dt, dens, v, a, f, f32, use32, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, int N_int, int N, double mass,
+                             double dt, CD3 u, CD3 f, CF3 f32, int use32, CD3 fo, CD3 vho, CD3 b,
+                             double *partials, DevScalars *S, double tol)
+{
+    k_adr_reduce<<<blocks, threads, 0, s>>>(dim, N_int, N, mass, dt, u, f, f32, use32, fo, vho, b, partials, S,
+                                            tol);
+    return hipGetLastError();
+}
+
+hipError_t launch_adr_update(hipStream_t s, int dim, int N_int, double mass, double dt, CD3 f, CF3 f32,
+                             int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S)
+{
+    if (N_int <= 0) return hipSuccess;
+    k_adr_update<<<nblk(N_int, 256), 256, 0, s>>>(dim, N_int, mass, dt, f, f32, use32, b, vh, v, vho, u, fo, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done)
+{
+    k_set_ct<<<1, 1, 0, s>>>(S, ct, clear_done);
+    return hipGetLastError();
+}
+
+hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u, D3 v,
+                          pd_cyl cyl, double dt, double dens, double vol, double *partials, DevScalars *S)
+{
+    k_contact<<<blocks, threads, 0, s>>>(N, dmg, x, u, v, cyl, dt, dens, vol, partials, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double *v, int64_t n,
+                            double *partials, DevScalars *S)
+{
+    k_field_sum<<<blocks, threads, 0, s>>>(v, n, partials, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_displace(hipStream_t s, int N_int, int n_top, int n_bot, double dtop, double dbot,
+                                 double *uy)
+{
+    int n = n_top + n_bot;
+    if (n <= 0) return hipSuccess;
+    k_strip_displace<<<nblk(n, 256), 256, 0, s>>>(N_int, n_top, n_bot, dtop, dbot, uy);
+    return hipGetLastError();
+}
+
+hipError_t launch_f32_to_f64(hipStream_t s, const float *a, double *b, int64_t n)
+{
+    if (n <= 0) return hipSuccess;
+    k_f32_to_f64<<<nblk(n, 256), 256, 0, s>>>(a, b, n);
+    return hipGetLastError();
+}

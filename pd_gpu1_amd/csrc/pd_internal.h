@@ -1,0 +1,183 @@
+// pd_internal.h -- handle layout and launcher prototypes shared by the HIP
+// translation units of libpd_gpu1_amd.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/pd_gpu.h"
+
+// Three component pointers passed by value to kernels (SoA [dim][N]).
+struct D3 { double *p[3]; };
+struct CD3 { const double *p[3]; };
+struct F3 { float *p[3]; };
+struct CF3 { const float *p[3]; };
+
+static inline CD3 cd3(const D3 &a) { return CD3{{a.p[0], a.p[1], a.p[2]}}; }
+static inline CF3 cf3(const F3 &a) { return CF3{{a.p[0], a.p[1], a.p[2]}}; }
+
+// Device scalars of one handle (one allocation, written by kernels).
+struct DevScalars {
+    double c;          // ADR damping coefficient of the last iteration
+    double resid;      // relative residual of the state before the last update
+    double sums[8];    // cn1, cn2, rnum, rden, react x/y/z, spare
+    double cyl[12];    // cylinder vel[3], disp[3], acc[3], force[3]
+    int nonfinite;     // set when a reduction meets NaN/Inf
+    int ct_counter;    // ADR iteration index handed to the next update
+    int ct_use;        // ADR iteration index of the current update
+    int done;          // ADR stop flag (device-side termination)
+    int iters;         // ADR iterations executed since the flag was cleared
+    int capacity;      // neighbour-capacity overflow flag
+    unsigned ticket;   // last-block ticket of the fused reduction
+    unsigned pad[3];
+};
+
+struct pd_handle {
+    pd_params P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    int N = 0, N_int = 0;
+    int64_t Nb = 0;
+
+    // particles (FP64 SoA)
+    D3 x{}, u{}, v{}, a{}, b{}, f{}, vh{}, vho{}, fo{};
+    F3 f32{};                 // mixed-mode force
+    double *m = nullptr, *theta = nullptr, *dmg = nullptr;
+    float *minv32 = nullptr, *theta32 = nullptr;  // mixed LPS: 1/m and dilatation
+    double *react[3] = {nullptr, nullptr, nullptr};
+
+    // bonds (CSR)
+    int32_t *row_ptr = nullptr;
+    int32_t *col = nullptr;
+    uint32_t *fail = nullptr;
+    double *idist = nullptr, *fac = nullptr, *scr = nullptr, *kappa = nullptr;
+    double *w = nullptr, *e = nullptr;
+    float *xi32 = nullptr, *coef32 = nullptr;
+    bool coef_dirty = true;   // kappa / coef32 need recomputation
+    bool have_scr = false;
+    bool have_weights = false;
+
+    // neighbour-search scratch
+    double lo[3] = {0, 0, 0};
+    int ncell[3] = {1, 1, 1};
+
+    // reductions
+    int red_blocks = 16, red_threads = 256;
+    double *partials = nullptr;     // [red_blocks * 8]
+    DevScalars *S = nullptr;
+
+    // host mirror of the cylinder parameters
+    pd_cyl cyl{};
+    bool have_cyl = false;
+
+    // force launch shape (autotuned group size)
+    int group = 0;
+
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_force_ms = 0.0;
+
+    // captured step graph
+    hipGraphExec_t step_graph = nullptr;
+    int step_graph_kind = -1;
+    double graph_tol = -2.0;
+};
+
+#define PD_TRY_HIP(h, expr)                                                       \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(_e);         \
+            return PD_E_HIP;                                                      \
+        }                                                                         \
+    } while (0)
+
+// ---------------------------------------------------------------- launchers
+// Each returns hipSuccess or the launch error.  Defined next to the kernels.
+
+// k_neighbor.hip
+int build_neighbors_impl(pd_handle *h);
+hipError_t launch_cell_ids(hipStream_t s, int dim, int N, CD3 x, const double *lo, double cs,
+                           const int *ncell, uint32_t *cell_of, uint32_t *idx);
+hipError_t launch_cell_bounds(hipStream_t s, int N, const uint32_t *sorted_cell, int ncells,
+                              int32_t *cell_start);
+hipError_t launch_neigh_count(hipStream_t s, int dim, int N, CD3 x, const double *lo, double cs,
+                              const int *ncell, const int32_t *cell_start, const uint32_t *items,
+                              double d2, int32_t *cnt);
+hipError_t launch_neigh_fill(hipStream_t s, int dim, int N, CD3 x, const double *lo, double cs,
+                             const int *ncell, const int32_t *cell_start, const uint32_t *items,
+                             double d2, const int32_t *row_ptr, int32_t *col);
+hipError_t launch_row_sort(hipStream_t s, int N, const int32_t *row_ptr, int32_t *col, int max_row);
+
+// k_setup.hip
+hipError_t launch_vol_corr(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col, CD3 x,
+                           double delta, double dx, double *idist, double *fac);
+hipError_t launch_surface_probe(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col,
+                                CD3 x, int axis, const double *idist, const double *fac, double bc,
+                                double vol, double sedload, double *fncst);
+hipError_t launch_surface_scr(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col,
+                              CD3 x, const double *idist, const double *fncst, double *scr);
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, int64_t n, uint32_t v);
+hipError_t launch_fill_f64(hipStream_t s, double *p, int64_t n, double v);
+hipError_t launch_crack_2d(hipStream_t s, int N_int, const int32_t *rp, const int32_t *col, CD3 x,
+                           double clength, double loc_x, double loc_y, int vertical, double t,
+                           double sq1, double half_s, uint32_t *fail);
+hipError_t launch_crack_kw(hipStream_t s, int N_int, const int32_t *rp, const int32_t *col, CD3 x,
+                           double rad, uint32_t *fail);
+hipError_t launch_lps_weights(hipStream_t s, int N, const int32_t *rp, const double *idist,
+                              const double *fac, double delta, double vol, double *w, double *m);
+hipError_t launch_pmb_coef(hipStream_t s, int64_t nb, const double *fac, const double *scr, double vol,
+                           double bc, const double *idist, double *kappa, float *xi32, float *coef32);
+hipError_t launch_lps_coef(hipStream_t s, int64_t nb, const double *w, const double *fac, double vol,
+                           const double *idist, float *xi32, float *coef32, int N, const double *m,
+                           float *minv32);
+hipError_t launch_damage(hipStream_t s, int N_int, const int32_t *rp, const uint32_t *fail,
+                         const double *fac, double vol, double *dmg);
+
+// k_force.hip
+hipError_t launch_pmb_fp64(hipStream_t s, const int *stop, int dim, int n_rows, const int32_t *rp, const int32_t *col,
+                           const double *kappa, const double *idist, uint32_t *fail, CD3 x, CD3 u,
+                           D3 f, double s0, double ymin);
+hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, int dim, int group, int n_rows, const int32_t *rp,
+                            const int32_t *col, const float *xi32, const float *kap32, uint32_t *fail,
+                            CD3 x, CD3 u, F3 f, double s0, double window, double ymin);
+hipError_t launch_lps_theta_fp64(hipStream_t s, const int *stop, int dim, int N, const int32_t *rp, const int32_t *col,
+                                 CD3 x, CD3 u, const double *idist, const double *fac, const double *w,
+                                 const double *m, const uint32_t *fail, double vol, double pratio,
+                                 int skip_broken, double *e, double *theta);
+hipError_t launch_lps_force_fp64(hipStream_t s, const int *stop, int dim, int N_int, const int32_t *rp, const int32_t *col,
+                                 CD3 x, CD3 u, const double *idist, const double *fac, const double *w,
+                                 const double *m, const double *theta, const double *e, double vol,
+                                 double pratio, double K, double G, double s0, double ymin,
+                                 uint32_t *fail, D3 f);
+hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, int dim, int group, int N, const int32_t *rp,
+                                  const int32_t *col, const float *xi32, const float *om32,
+                                  const uint32_t *fail, CD3 x, CD3 u, const float *m32, float ctheta,
+                                  int skip_broken, float *theta32);
+hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, int dim, int group, int N_int, const int32_t *rp,
+                                  const int32_t *col, const float *xi32, const float *om32, uint32_t *fail,
+                                  CD3 x, CD3 u, const float *m32, const float *theta32, float A, float B,
+                                  double s0, double window, double ymin, F3 f);
+
+// k_integrate.hip
+hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
+                          D3 v, CD3 a, D3 u, double *cyl_state);
+hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f,
+                          CF3 f32, int use32, CD3 b);
+hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, int N_int, int N,
+                             double mass, double dt, CD3 u, CD3 f, CF3 f32, int use32, CD3 fo, CD3 vho,
+                             CD3 b, double *partials, DevScalars *S, double tol);
+hipError_t launch_adr_update(hipStream_t s, int dim, int N_int, double mass, double dt, CD3 f,
+                             CF3 f32, int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo,
+                             DevScalars *S);
+hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u,
+                          D3 v, pd_cyl cyl, double dt, double dens, double vol, double *partials,
+                          DevScalars *S);
+hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done);
+hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double *v, int64_t n,
+                            double *partials, DevScalars *S);
+hipError_t launch_strip_displace(hipStream_t s, int N_int, int n_top, int n_bot, double dtop,
+                                 double dbot, double *uy);
+hipError_t launch_f32_to_f64(hipStream_t s, const float *a, double *b, int64_t n);

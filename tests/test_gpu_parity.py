@@ -1,0 +1,283 @@
+"""GPU parity: the HIP engine against the FP64 oracle on the same inputs.
+
+Integer/index results (CSR, broken-bond sets) must be bit-exact.  FP64-mode
+PMB, Verlet, ADR and contact are bit-exact too (same operand order, no FMA,
+same fixed reduction tree).  LPS in FP64 mode uses the device exp for the
+influence weight, which may differ from glibc exp in the last bit, so LPS
+fields are compared at 1e-12 relative (the 1e-10 target with margin).  Mixed
+mode is compared at the FP32 tolerances stated in each test.
+"""
+import numpy as np
+import pytest
+
+import pd_gpu1_amd as pd
+from pd_gpu1_amd import _lib, scenarios
+from oracle import oracle as O
+from oracle.runner import OracleModel
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_for(sc, brute=False):
+    m = OracleModel(scenarios.params_dict(sc.p), sc.x, sc.y, sc.z, sc.n_int)
+    m.build_neighbors(brute=brute)
+    return m
+
+
+def _relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ------------------------------------------------------------------ setup
+def test_csr_and_volcorr_c0_bit_exact():
+    sc = scenarios.c0_plate_pmb()
+    print("case: c0 plate N=%d" % len(sc.x), flush=True)
+    with sc.make() as prob:
+        rp, col = prob.csr()
+        idist, fac = prob.get("IDIST"), prob.get("FAC")
+        mask = prob.fail_mask()
+    m = _oracle_for(sc, brute=True)
+    m.init_state()
+    m.set_crack_2d(0.01, 0.0, 0.0, 0.0)
+    assert len(col) == 289420
+    assert np.array_equal(rp, m.row_ptr) and np.array_equal(col, m.col)
+    assert np.array_equal(idist, m.idist) and np.array_equal(fac, m.fac)
+    assert np.array_equal(mask, m.fail)
+    assert (mask == 0).sum() > 0
+
+
+def test_csr_kw_and_surface_bit_exact():
+    sc = scenarios.c1_kalthoff_winkler(delta_factor=3.015)
+    print("case: kw N=%d" % len(sc.x), flush=True)
+    with sc.make() as prob:
+        rp, col = prob.csr()
+        scr = prob.get("SCR")
+        mask = prob.fail_mask()
+    m = _oracle_for(sc)
+    m.surface_correct()
+    m.init_state()
+    m.set_notches_kw(0.025)
+    assert len(sc.x) == 180873 and len(col) == 18809054
+    assert np.array_equal(rp, m.row_ptr) and np.array_equal(col, m.col)
+    assert np.array_equal(scr, m.scr)
+    assert np.array_equal(mask, m.fail)
+
+
+def test_csr_knife_edge_delta_3dx():
+    # delta = 3 dx puts 30 lattice offsets exactly on the horizon; the FP64
+    # predicate decides them, identically on both sides.
+    sc = scenarios.c1_kalthoff_winkler(delta_factor=3.0, width=0.005)
+    with sc.make() as prob:
+        rp, col = prob.csr()
+    m = _oracle_for(sc)
+    assert np.array_equal(rp, m.row_ptr) and np.array_equal(col, m.col)
+
+
+def test_csr_jittered_block_bit_exact():
+    sc = scenarios.c2_lps_block(nx=40, ny=36, nz=20, precision=_lib.PD_PREC_FP64)
+    with sc.make() as prob:
+        rp, col = prob.csr()
+    m = _oracle_for(sc)
+    assert np.array_equal(rp, m.row_ptr) and np.array_equal(col, m.col)
+
+
+def test_capacity_error_is_reported():
+    sc = scenarios.c0_plate_pmb(scale=0.2)
+    p = sc.p
+    p.max_neighbors = 8
+    prob = pd.Problem(p, 0)
+    prob.set_positions(sc.x, sc.y, None, n_int=sc.n_int)
+    with pytest.raises(pd.PdError) as ei:
+        prob.build_neighbors()
+    assert ei.value.code == _lib.PD_E_CAPACITY
+    prob.close()
+
+
+# ------------------------------------------------------------------- force
+def test_pmb2d_force_bit_exact_random_states():
+    sc = scenarios.c0_plate_pmb()
+    m = _oracle_for(sc)
+    m.init_state()
+    m.set_crack_2d(0.01, 0.0, 0.0, 0.0)
+    rng = np.random.default_rng(7)
+    with sc.make() as prob:
+        for trial in range(3):
+            u = rng.uniform(-4e-6, 4e-6, size=(2, len(sc.x)))
+            prob.set("U", u)
+            m.U = [u[0].copy(), u[1].copy()]
+            prob.force()
+            m.force()
+            assert np.array_equal(prob.get("F"), m.stack("F")), trial
+            assert np.array_equal(prob.fail_mask(), m.fail), trial
+        assert (m.fail == 0).sum() > 400  # some bonds broke during the trials
+
+
+def test_c0_verlet_run_bit_exact_checkpoints():
+    sc = scenarios.c0_plate_pmb(steps=1000)
+    m = _oracle_for(sc)
+    m.init_state()
+    m.set_crack_2d(0.01, 0.0, 0.0, 0.0)
+    with sc.make() as prob:
+        done = 0
+        for chk in (1, 250, 500, 1000):
+            prob.run_verlet(chk - done)
+            for _ in range(chk - done):
+                m.step()
+            done = chk
+            print("c0 checkpoint", chk, flush=True)
+            assert np.array_equal(prob.get("U"), m.stack("U")), chk
+            assert np.array_equal(prob.get("V"), m.stack("V")), chk
+            assert np.array_equal(prob.fail_mask(), m.fail), chk
+        prob.damage()
+        m.damage()
+        assert np.array_equal(prob.get("DMG")[: sc.n_int], m.dmg[: sc.n_int])
+        assert (m.fail == 0).sum() > (m.fail == 0).sum() - 1
+
+
+def test_kw_run_with_contact_bit_exact():
+    sc = scenarios.c1_kalthoff_winkler(steps=80, width=0.005)
+    m = _oracle_for(sc)
+    m.surface_correct()
+    m.init_state()
+    m.set_notches_kw(0.025)
+    m.cylinder_init(sc.cyl)
+    with sc.make() as prob:
+        prob.run_verlet(80)
+        for _ in range(80):
+            m.step()
+        assert np.array_equal(prob.get("U"), m.stack("U"))
+        assert np.array_equal(prob.get("V"), m.stack("V"))
+        assert np.array_equal(prob.fail_mask(), m.fail)
+        cs = prob.cylinder_state()
+        assert np.array_equal(cs["vel"], m.cst["vel"]) and np.array_equal(cs["disp"], m.cst["disp"])
+        assert m.cst["vel"][1] > -32.0  # the cylinder has been slowed by contact
+
+
+def test_lps2d_crack_grow_fp64():
+    # Solve_Plate_Crack_Grow.cu at reduced size: 2D LPS with strips.
+    dx = 1e-4
+    p, (nx, ny, _), _n = pd.params(2, _lib.PD_MODEL_LPS, 1.3367e-8, dx, thick=1e-4, length=0.006, height=0.006,
+                                   emod=192e9, pratio=0.3, dens=7850.0, s0=0.04472)
+    p.n_top = p.n_bot = 3 * nx
+    p.strip_velocity = 20.0
+    x, y = pd.gen_plate_strips(nx, ny, dx)
+    prob = pd.Problem(p, 0)
+    prob.set_positions(x, y, None, n_int=nx * ny)
+    prob.build_neighbors()
+    prob.init_state()
+    prob.set_crack_2d(0.002, 0.0, 0.0, 0.0)
+    prob.lps_weights()
+    m = OracleModel(scenarios.params_dict(p), x, y, None, nx * ny)
+    m.build_neighbors()
+    m.init_state()
+    m.set_crack_2d(0.002, 0.0, 0.0, 0.0)
+    m.lps_weights()
+    assert _relerr(prob.get("W"), m.w) < 4e-16 and _relerr(prob.get("M")[: len(x)], m.m) < 1e-14
+    prob.run_verlet(300)
+    for _ in range(300):
+        m.step()
+    assert _relerr(prob.get("U"), m.stack("U")) < 1e-12
+    assert np.array_equal(prob.fail_mask(), m.fail)
+    prob.close()
+
+
+def test_lps3d_block_fp64():
+    sc = scenarios.c2_lps_block(nx=24, ny=20, nz=16, precision=_lib.PD_PREC_FP64)
+    m = _oracle_for(sc)
+    m.init_state()
+    m.lps_weights()
+    m.V = [a.copy() for a in sc.v0]
+    with sc.make() as prob:
+        prob.run_verlet(100)
+        for _ in range(100):
+            m.step()
+        assert _relerr(prob.get("U"), m.stack("U")) < 1e-12
+        assert _relerr(prob.get("THETA"), m.theta) < 1e-10
+        assert np.array_equal(prob.fail_mask(), m.fail)
+
+
+# --------------------------------------------------------------------- ADR
+def test_adr_plate_tensile_bit_exact():
+    sc = scenarios.plate_tensile(dx=0.02)
+    m = _oracle_for(sc)
+    m.surface_correct()
+    m.init_state()
+    bx = np.zeros(len(sc.x))
+    nx, ny, _ = sc.nxyz
+    for i in range(ny):
+        bx[i * nx] = -0.001 * sc.p.emod / sc.p.dx
+        bx[i * nx + nx - 1] = 0.001 * sc.p.emod / sc.p.dx
+    m.B[0][:] = bx
+    with sc.make() as prob:
+        assert np.array_equal(prob.get("SCR"), m.scr)
+        assert np.array_equal(prob.get("B")[0], bx)
+        for ct in range(200):
+            c_g, r_g = prob.adr_iterate(ct)
+            c_o, r_o = m.adr_iterate(ct)
+            assert c_g == c_o and r_g == r_o, ct
+        assert np.array_equal(prob.get("U"), m.stack("U"))
+        # device-side convergence loop continues from the same state
+        it, r, ok = prob.adr_solve(200, 50, 0.0)
+        assert it == 50 and not ok
+
+
+def test_reduce_sum_matches_oracle_tree():
+    sc = scenarios.c0_plate_pmb(scale=0.5)
+    rng = np.random.default_rng(3)
+    with sc.make() as prob:
+        u = rng.standard_normal((2, len(sc.x)))
+        prob.set("U", u)
+        for comp in (0, 1):
+            assert prob.reduce_sum("U", comp) == O.sum_tree(u[comp], sc.p.reduce_lanes)
+
+
+def test_checkpoint_resume_is_bit_identical(tmp_path):
+    sc = scenarios.c0_plate_pmb(scale=0.4)
+    with sc.make() as a:
+        a.run_verlet(40)
+        a.checkpoint_write(tmp_path / "c.bin", 40)
+        a.run_verlet(40)
+        ua, fa = a.get("U"), a.fail_mask()
+    with sc.make() as b:
+        assert b.checkpoint_read(tmp_path / "c.bin") == 40
+        b.run_verlet(40)
+        assert np.array_equal(b.get("U"), ua) and np.array_equal(b.fail_mask(), fa)
+
+
+# ------------------------------------------------------------------- mixed
+def test_mixed_pmb3d_force_and_breaks():
+    sc = scenarios.pmb_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+    m = _oracle_for(sc)
+    m.init_state()
+    rng = np.random.default_rng(11)
+    u = np.stack([2.05e-3 * sc.x, -4e-4 * sc.y, 1e-3 * sc.z]) + rng.uniform(-2e-9, 2e-9, (3, len(sc.x)))
+    with sc.make() as prob:
+        prob.set("U", u)
+        m.U = [a.copy() for a in u]
+        prob.force()
+        m.force()
+        f_g, f_o = prob.get("F"), m.stack("F")
+        scale = np.max(np.abs(f_o))
+        assert np.max(np.abs(f_g - f_o)) / scale < 2e-5
+        assert np.array_equal(prob.fail_mask(), m.fail)
+        assert (m.fail == 0).sum() > 0
+
+
+def test_mixed_lps3d_force():
+    sc = scenarios.c2_lps_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+    m = _oracle_for(sc)
+    m.init_state()
+    m.lps_weights()
+    rng = np.random.default_rng(12)
+    u = np.stack([1.1e-3 * sc.x, 3e-4 * sc.y, 2.05e-3 * sc.z]) + rng.uniform(-2e-9, 2e-9, (3, len(sc.x)))
+    with sc.make() as prob:
+        prob.set("U", u)
+        m.U = [a.copy() for a in u]
+        prob.force()
+        m.force()
+        assert _relerr(prob.get("THETA"), m.theta) < 2e-5
+        f_g, f_o = prob.get("F")[:, : sc.n_int], m.stack("F")[:, : sc.n_int]
+        assert np.max(np.abs(f_g - f_o)) / np.max(np.abs(f_o)) < 5e-5
+        assert np.array_equal(prob.fail_mask(), m.fail)
+        assert (m.fail == 0).sum() > 0
