@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Throughput of the peridynamic time step on one MI355X.
+
+Workload (default): the 3D state-based LPS block of 200 x 200 x 100 particles
+(0.1 x 0.1 x 0.05 m at dx = 0.5 mm, seeded +-1% dx jitter, delta = 3.015 dx,
+E = 70 GPa, nu = 0.33, rho = 2700 kg/m^3, s0 = 0.002, v_z(0) = 1e3 z,
+dt = 2e-8 s) in mixed precision: FP32 bond math with compensated sums, FP64
+positions, displacements and near-critical bonds.  One step = Verlet half 1,
+dilatation pass, force pass, Verlet half 2, replayed from a captured HIP graph.
+
+value = directed bonds x steps / wall time of the timed steps (bond-updates/s,
+each LPS bond-update covering both sweeps); inputs are resident in HBM before
+the timed region starts.  The roofline object reports the dominant kernel
+(the LPS force pass): algorithmic bytes per launch / its mean HIP-event
+duration against the 8 TB/s HBM3E peak.  cpu_baseline times the FP64 oracle on
+the host cores over a bounded sub-block of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def lps_pass_bytes(nb, n, n_int, dim=3):
+    """Algorithmic bytes of the mixed LPS passes.
+
+    Per bond and pass: col 4 + xi 4 + omega 4 + fail bit 1/8 = 12.125 B.
+    Dilatation pass per particle: x0 8D + u 8D + 1/m 4 + theta write 4 + row_ptr 4.
+    Force pass per particle: x0 8D + u 8D + theta 4 + 1/m 4 + f write 4D + row_ptr 4.
+    """
+    theta = 12.125 * nb + (16 * dim + 12) * n
+    force = 12.125 * nb + (20 * dim + 12) * n_int
+    return theta, force
+
+
+def pmb_bytes(nb, n, dim=3):
+    """Mixed PMB: col 4 + xi 4 + kappa 4 + fail 1/8 per bond; x0 8D + u 8D + f 4D + row_ptr 4."""
+    return 12.125 * nb + (20 * dim + 4) * n
+
+
+def load_traffic(kernel_key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data.get(kernel_key)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(sample_dims=(200, 100, 50), evals=3):
+    """FP64 oracle (OpenMP) on a sub-block of the same lattice and physics."""
+    import numpy as np
+    from oracle.runner import OracleModel
+    from pd_gpu1_amd import scenarios
+
+    nx, ny, nz = sample_dims
+    sc = scenarios.c2_lps_block(nx=nx, ny=ny, nz=nz)
+    m = OracleModel(scenarios.params_dict(sc.p), sc.x, sc.y, sc.z, sc.n_int)
+    m.build_neighbors()
+    m.init_state()
+    m.lps_weights()
+    m.V = [a.copy() for a in sc.v0]
+    nb = len(m.col)
+    m.step()  # warm caches and page in
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        m.step()
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    del np
+    return {
+        "value": nb * evals / dt,
+        "unit": "bond-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP64 oracle, {nx}x{ny}x{nz} sub-block of the same jittered LPS lattice "
+                  f"({len(sc.x)} particles, {nb} bonds), {evals} full Verlet+LPS steps, {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--reps", type=int, default=20, help="launches per pass for the kernel timing")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = built-in choice)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this engine runs one specimen on one GPU; multi-GPU runs are not part of "
+              "this build (see the sharding design).  Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+
+    from pd_gpu1_amd import _lib, scenarios
+
+    t_setup = time.perf_counter()
+    sc = scenarios.c2_lps_block(precision=_lib.PD_PREC_MIXED)
+    prob = sc.make(device=0)
+    if args.group:
+        prob.set_group(args.group)
+    prob.sync()
+    t_setup = time.perf_counter() - t_setup
+    N, N_int, nb = prob.N, prob.N_int, prob.num_bonds
+    print(f"[bench] setup {t_setup:.2f} s: N={N} Nb={nb}", file=sys.stderr, flush=True)
+
+    prob.run_verlet(args.warmup)
+    prob.sync()
+    t0 = time.perf_counter()
+    prob.run_verlet(args.steps)
+    prob.sync()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / max(args.steps, 1)
+    value = nb * args.steps / wall
+    print(f"[bench] {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+
+    ms_theta, ms_force = prob.time_force_passes(args.reps)
+    b_theta, b_force = lps_pass_bytes(nb, N, N_int)
+    gbs_theta = b_theta / (ms_theta * 1e-3) / 1e9
+    gbs_force = b_force / (ms_force * 1e-3) / 1e9
+    broken = int((prob.fail_mask() == 0).sum())
+    prob.close()
+
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_lps_force_mixed<3,%d>" % (args.group or 64),
+        "achieved": round(gbs_force, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs_force / HBM_PEAK_GBS, 4),
+        "traffic": load_traffic("lps_force_mixed"),
+        "bytes_per_launch": b_force,
+        "ms_per_launch": ms_force,
+        "theta_pass": {"achieved": round(gbs_theta, 1), "frac": round(gbs_theta / HBM_PEAK_GBS, 4),
+                       "bytes_per_launch": b_theta, "ms_per_launch": ms_theta},
+    }
+    out = {
+        "metric": "bond-updates/s & %HBM peak on 1 MI355X",
+        "value": value,
+        "unit": "bond-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded jittered lattice)",
+        "config": {"workload": "3D LPS block 200x200x100 (4M particles), dx=0.5mm, delta=3.015dx, "
+                               "mixed precision, 200 Verlet steps at dt=2e-8",
+                   "particles": N, "bonds": nb, "parallelism": "single GPU",
+                   "force_bond_updates_per_s": nb / ((ms_theta + ms_force) * 1e-3),
+                   "broken_bonds_after_run": broken},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -195,8 +195,45 @@ int pd_save_text(pd_handle *h, const char *path, int layout, int nx, int ny, int
 int pd_checkpoint_write(pd_handle *h, const char *path, int64_t step);
 int pd_checkpoint_read(pd_handle *h, const char *path, int64_t *step);
 
-/* ---- timing: device-event time of the last force launch(es) ------------- */
-int pd_last_force_ms(pd_handle *h, double *ms);
+/* ---- batched quasi-static crack growth (no reference counterpart; built
+ *      from the ADR path of solve_plate_tensile, Solve_Plate_Tensile.cu:106-131,
+ *      the strip layout of Coord.cu:47-66 and the crack of
+ *      Base_Function.cu:83-123) ---------------------------------------------
+ * Independent square 2D PMB plates with 3 strip rows above and below and a
+ * centred pre-crack, relaxed by ADR in one persistent launch: one workgroup
+ * per specimen, specimens handed out by a device work queue.  All specimens
+ * share the material, dx, dt and mass of *p. */
+typedef struct pd_specimen {
+    int32_t nx, ny;            /* plate lattice (strips add 3*nx above and below) */
+    double crack_len;          /* centred crack length (m); <= 0: none */
+    double crack_angle;        /* crack angle (rad) */
+} pd_specimen;
+typedef struct pd_batch pd_batch;
+int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int device, pd_batch **out);
+int pd_batch_destroy(pd_batch *b);
+const char *pd_batch_last_error(pd_batch *b);
+int pd_batch_info(pd_batch *b, int64_t *n_particles, int64_t *n_bonds, int *nspec);
+int pd_batch_spec_info(pd_batch *b, int spec, int *n, int *n_int, int64_t *nb);
+/* One load increment: top strips += du, bottom strips -= du, then ADR from
+ * rest (first iteration undamped, Integrate.cu:65-76) until the residual
+ * ||F||(free) / ||F||(strips) <= tol or max_iters updates.  block = threads
+ * per workgroup (256, 512, 1024), grid = workgroups (0 = fill the device).
+ * iters_out/resid_out[nspec] receive the updates done and the final residual. */
+int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int block, int grid,
+                       int32_t *iters_out, double *resid_out, double *ms_out);
+int pd_batch_get_field(pd_batch *b, int spec, int field, double *out);       /* [2][n] */
+int pd_batch_set_field(pd_batch *b, int spec, int field, const double *in);
+int pd_batch_get_csr(pd_batch *b, int spec, int32_t *row_ptr, int32_t *col, uint32_t *fail_words);
+
+/* ---- timing and launch shape (no reference counterpart: the reference times
+ *      whole runs with gettimeofday, Base_Function.cu:7-12) ---------------- */
+int pd_last_force_ms(pd_handle *h, double *ms);       /* events around the last pd_force */
+/* Average HIP-event time per pass over reps evaluations: ms_out[0] = PMB kernel
+ * or LPS dilatation pass, ms_out[1] = LPS force pass. */
+int pd_time_force_passes(pd_handle *h, int reps, double *ms_out);
+/* Lanes per particle of the mixed kernels: 4, 8, 16, 32, 64, or 0 for the
+ * built-in choice from the mean row length. */
+int pd_set_group(pd_handle *h, int group);
 int pd_sync(pd_handle *h);
 
 #ifdef __cplusplus

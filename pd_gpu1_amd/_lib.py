@@ -29,7 +29,9 @@ EXPORTS = (
     "pd_adr_iterate", "pd_adr_solve", "pd_strip_displace", "pd_reduce_sum", "pd_damage",
     "pd_get_field", "pd_set_field", "pd_get_fail_bits", "pd_set_fail_bits", "pd_cylinder_init",
     "pd_cylinder_get", "pd_contact_step", "pd_save_text", "pd_checkpoint_write", "pd_checkpoint_read",
-    "pd_last_force_ms", "pd_sync",
+    "pd_last_force_ms", "pd_time_force_passes", "pd_set_group", "pd_sync",
+    "pd_batch_create", "pd_batch_destroy", "pd_batch_last_error", "pd_batch_info", "pd_batch_spec_info",
+    "pd_batch_increment", "pd_batch_get_field", "pd_batch_set_field", "pd_batch_get_csr",
 )
 
 
@@ -42,6 +44,10 @@ class PdParams(C.Structure):
 
 class PdCyl(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("rad", "mass", "height", "v", "x", "y", "z")]
+
+
+class PdSpecimen(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("crack_len", C.c_double), ("crack_angle", C.c_double)]
 
 
 class PdError(RuntimeError):
@@ -106,13 +112,25 @@ def lib():
             "pd_checkpoint_write": [H, C.c_char_p, C.c_int64],
             "pd_checkpoint_read": [H, C.c_char_p, C.POINTER(C.c_int64)],
             "pd_last_force_ms": [H, D],
+            "pd_time_force_passes": [H, C.c_int, D],
+            "pd_set_group": [H, C.c_int],
             "pd_sync": [H],
+            "pd_batch_create": [C.POINTER(PdParams), C.POINTER(PdSpecimen), C.c_int, C.c_int, C.POINTER(H)],
+            "pd_batch_destroy": [H],
+            "pd_batch_last_error": [H],
+            "pd_batch_info": [H, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)],
+            "pd_batch_spec_info": [H, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)],
+            "pd_batch_increment": [H, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, I32, D, D],
+            "pd_batch_get_field": [H, C.c_int, C.c_int, D],
+            "pd_batch_set_field": [H, C.c_int, C.c_int, D],
+            "pd_batch_get_csr": [H, C.c_int, I32, I32, U32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
         L.pd_last_error.restype = C.c_char_p
+        L.pd_batch_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 

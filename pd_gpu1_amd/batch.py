@@ -1,0 +1,123 @@
+"""Batched quasi-static crack growth: many 2D PMB plates relaxed by ADR in one
+persistent launch per load increment (see include/pd_gpu.h, pd_batch_*)."""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import PdError, PdSpecimen, dptr, lib
+from .engine import params
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(state):
+    """SplitMix64 (Steele, Lea, Flood 2014): returns (value, new_state)."""
+    state = (state + 0x9E3779B97F4A7C15) & MASK64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return z ^ (z >> 31), state
+
+
+def uniform01(state):
+    v, state = splitmix64(state)
+    return (v >> 11) * (1.0 / 9007199254740992.0), state
+
+
+def plate_family(count, seed_base=0x5EED0003, n_min=40, n_max=200, dx=0.5e-3):
+    """Seeded specimens of the batched family: side n ~ U{n_min..n_max}
+    particles, crack length U(0.1, 0.5) * n * dx, angle U(-30, 30) degrees.
+    Specimen k draws from its own SplitMix64 stream seeded seed_base + k."""
+    specs = []
+    for k in range(count):
+        st = seed_base + k
+        u, st = uniform01(st)
+        n = n_min + min(int(u * (n_max - n_min + 1)), n_max - n_min)
+        u, st = uniform01(st)
+        frac = 0.1 + 0.4 * u
+        u, st = uniform01(st)
+        ang = math.radians(-30.0 + 60.0 * u)
+        specs.append((n, n, frac * n * dx, ang))
+    return specs
+
+
+def plate_params(dx=0.5e-3):
+    """2D PMB plate constants of the family: E = 192 GPa, nu = 1/3, rho = 8000,
+    s0 = 0.004, t = dx, ADR fictitious dt = 1 with the mass of Base.cuh:150."""
+    p, _, _ = params(2, _lib.PD_MODEL_PMB, 1.0, dx, thick=dx, length=0.02, height=0.02, emod=192e9,
+                     pratio=1.0 / 3.0, dens=8000.0, s0=0.004)
+    return p
+
+
+class Batch:
+    """A pd_batch handle: independent plates on one GPU."""
+
+    def __init__(self, p, specs, device=0):
+        arr = (PdSpecimen * len(specs))(*[PdSpecimen(int(a), int(b), float(c), float(d)) for a, b, c, d in specs])
+        self._h = C.c_void_p()
+        rc = lib().pd_batch_create(C.byref(p), arr, len(specs), device, C.byref(self._h))
+        if rc != _lib.PD_OK:
+            raise PdError(rc, "pd_batch_create failed")
+        self.nspec = len(specs)
+        self.specs = list(specs)
+
+    def close(self):
+        if self._h:
+            lib().pd_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, rc):
+        if rc != _lib.PD_OK:
+            raise PdError(rc, (lib().pd_batch_last_error(self._h) or b"").decode())
+
+    def info(self):
+        n, nb, ns = C.c_int64(), C.c_int64(), C.c_int()
+        self._c(lib().pd_batch_info(self._h, C.byref(n), C.byref(nb), C.byref(ns)))
+        return n.value, nb.value, ns.value
+
+    def spec_info(self, s):
+        n, ni, nb = C.c_int(), C.c_int(), C.c_int64()
+        self._c(lib().pd_batch_spec_info(self._h, s, C.byref(n), C.byref(ni), C.byref(nb)))
+        return n.value, ni.value, nb.value
+
+    def increment(self, du, tol, max_iters, block=1024, grid=0):
+        its = np.zeros(self.nspec, dtype=np.int32)
+        res = np.zeros(self.nspec)
+        ms = C.c_double()
+        self._c(lib().pd_batch_increment(self._h, du, tol, max_iters, block, grid,
+                                         its.ctypes.data_as(_lib.I32), dptr(res), C.byref(ms)))
+        return its, res, ms.value
+
+    def get(self, s, field="U"):
+        n, _, _ = self.spec_info(s)
+        out = np.zeros(2 * n)
+        self._c(lib().pd_batch_get_field(self._h, s, _lib.FIELDS[field], dptr(out)))
+        return out.reshape(2, n)
+
+    def set(self, s, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._c(lib().pd_batch_set_field(self._h, s, _lib.FIELDS[field], dptr(a)))
+
+    def csr(self, s):
+        n, _, nb = self.spec_info(s)
+        rp = np.zeros(n + 1, dtype=np.int32)
+        col = np.zeros(nb, dtype=np.int32)
+        fw = np.zeros((nb + 31) // 32, dtype=np.uint32)
+        self._c(lib().pd_batch_get_csr(self._h, s, rp.ctypes.data_as(_lib.I32), col.ctypes.data_as(_lib.I32),
+                                       fw.ctypes.data_as(_lib.U32)))
+        mask = np.unpackbits(fw.view(np.uint8), bitorder="little")[:nb].astype(np.uint8)
+        return rp, col, mask

@@ -1,0 +1,184 @@
+// k_batch.hip -- batched quasi-static crack growth: many independent 2D PMB
+// plates relaxed by adaptive dynamic relaxation (ADR) in one persistent launch.
+//
+// One workgroup owns one specimen at a time and pulls the next specimen index
+// from a device work queue (an atomic counter), so 512 plates of 1.6k-40k
+// particles load-balance over the 256 CUs without host involvement.  Inside a
+// specimen the workgroup runs whole ADR iterations: PMB force over every row
+// (Force_Bond.cu:159-202 arithmetic, FP64, rows in CSR order), the damping
+// terms of kernel_cnn (Integrate.cu:14-34) and the residual reduced with the
+// fixed low-bit-first tree over T = blockDim lanes, the damping coefficient of
+// cal_cn (Integrate.cu:117-128), the convergence test, and the update of
+// kernel_integrate (Integrate.cu:62-105).  Workgroup barriers separate the
+// phases; no grid-wide synchronisation and no host round trip is needed
+// (the reference copies both sums to the host every iteration,
+// Reduce.cu:61-62, Integrate.cu:112-113).
+#include "pd_batch.h"
+
+namespace {
+
+__device__ __forceinline__ bool intact(const uint32_t *fail, int64_t k)
+{
+    return (fail[k >> 5] >> (k & 31)) & 1u;
+}
+
+template <int BT>
+__device__ __forceinline__ void block_canon4(double p[4], double (*sh)[BT / 64], double out[4])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) p[c] = p[c] + __shfl_down(p[c], off, 64);
+    if (lane == 0)
+        for (int c = 0; c < 4; ++c) sh[c][wave] = p[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int off = 1; off < BT / 64; off <<= 1)
+            for (int w = 0; w < BT / 64; w += 2 * off)
+                for (int c = 0; c < 4; ++c) sh[c][w] = sh[c][w] + sh[c][w + off];
+        for (int c = 0; c < 4; ++c) out[c] = sh[c][0];
+    }
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double tol, int max_iters,
+                                                  unsigned *queue, int32_t *iters_out, double *resid_out)
+{
+    __shared__ double sh[4][BT / 64];
+    __shared__ int spec_sh;
+    __shared__ double c_sh, r_sh;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) spec_sh = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        const int s = spec_sh;
+        __syncthreads();
+        if (s >= B.nspec) break;
+        const BatchSpec d = B.spec[s];
+        const int N = d.n, Ni = d.n_int;
+        const double *X = B.x + d.p_off, *Y = B.y + d.p_off;
+        double *UX = B.ux + d.p_off, *UY = B.uy + d.p_off;
+        double *FX = B.fx + d.p_off, *FY = B.fy + d.p_off;
+        double *VHX = B.vhx + d.p_off, *VHY = B.vhy + d.p_off;
+        double *VHOX = B.vhox + d.p_off, *VHOY = B.vhoy + d.p_off;
+        double *VX = B.vx + d.p_off, *VY = B.vy + d.p_off;
+        double *FOX = B.fox + d.p_off, *FOY = B.foy + d.p_off;
+        const int32_t *rp = B.row_ptr + d.rp_off;
+        const int32_t *col = B.col + d.b_off;
+        const double *kap = B.kappa + d.b_off, *idist = B.idist + d.b_off;
+        uint32_t *fail = B.fail + (d.b_off >> 5);
+        const double mass = B.mass, dt = B.dt, s0 = B.s0;
+        // displacement control of the strips (Solve_Plate_Crack_Grow.cu:20-27 layout)
+        for (int k = tid; k < d.n_top + d.n_bot; k += BT) UY[Ni + k] += k < d.n_top ? du : -du;
+        __syncthreads();
+        int it = 0;
+        double r = 0.0;
+        for (;;) {
+            // PMB force, every row (Force_Bond.cu:169)
+            for (int i = tid; i < N; i += BT) {
+                const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
+                double fx = 0.0, fy = 0.0;
+                for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                    if (!intact(fail, k)) continue;
+                    const int c = col[k];
+                    double d0 = X[c] + UX[c] - x0 - u0;
+                    double d1 = Y[c] + UY[c] - x1 - u1;
+                    double nl = sqrt(d0 * d0 + d1 * d1);
+                    double id = idist[k];
+                    double sv = (nl - id) / id;
+                    double ks = kap[k] * sv;
+                    fx += ks * d0 / nl;
+                    fy += ks * d1 / nl;
+                    if (sv > s0) atomicAnd(&fail[k >> 5], ~(1u << (k & 31)));
+                }
+                FX[i] = fx;
+                FY[i] = fy;
+            }
+            __syncthreads();
+            // damping and residual terms, lane t owns particles t, t+BT, ...
+            double p[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int i = tid; i < N; i += BT) {
+                double cn = 0.0, disp = 0.0, rn = 0.0, rd = 0.0;
+                if (i < Ni) {
+                    const double ux = UX[i], uy = UY[i], fx = FX[i], fy = FY[i];
+                    if (VHOX[i] != 0.0) cn = -1.0 * ux * ux * (fx / mass - FOX[i] / mass) / (dt * VHOX[i]);
+                    disp = ux * ux;
+                    if (VHOY[i] != 0.0) cn -= 1.0 * uy * uy * (fy / mass - FOY[i] / mass) / (dt * VHOY[i]);
+                    disp += uy * uy;
+                    rn = fx * fx;
+                    rn += fy * fy;
+                } else {
+                    rd = FX[i] * FX[i];
+                    rd += FY[i] * FY[i];
+                }
+                p[0] += cn;
+                p[1] += disp;
+                p[2] += rn;
+                p[3] += rd;
+            }
+            double tot[4];
+            block_canon4<BT>(p, sh, tot);
+            if (tid == 0) {
+                double cn1 = tot[0], cn2 = tot[1], cc;
+                if (cn2 != 0.0) {
+                    if (cn1 / cn2 > 0.0)
+                        cc = 2.0 * sqrt(cn1 / cn2);
+                    else
+                        cc = 0.0;
+                } else {
+                    cc = 0.0;
+                }
+                if (cc > 2.0) cc = 1.9;
+                c_sh = cc;
+                r_sh = tot[3] > 0.0 ? sqrt(tot[2]) / sqrt(tot[3]) : (tot[2] > 0.0 ? INFINITY : 0.0);
+            }
+            __syncthreads();
+            r = r_sh;
+            if (r <= tol || it >= max_iters || !(r == r)) break;
+            const double cc = c_sh;
+            for (int i = tid; i < Ni; i += BT) {
+                double h0, h1;
+                const double fx = FX[i], fy = FY[i];
+                if (it == 0) {
+                    h0 = 1.0 * dt / mass * (fx + 0.0) / 2.0;
+                    h1 = 1.0 * dt / mass * (fy + 0.0) / 2.0;
+                } else {
+                    h0 = ((2.0 - cc * dt) * VHOX[i] + 2.0 * dt / mass * (fx + 0.0)) / (2.0 + cc * dt);
+                    h1 = ((2.0 - cc * dt) * VHOY[i] + 2.0 * dt / mass * (fy + 0.0)) / (2.0 + cc * dt);
+                }
+                VHX[i] = h0;
+                VHY[i] = h1;
+                VX[i] = 0.5 * (VHOX[i] + h0);
+                VY[i] = 0.5 * (VHOY[i] + h1);
+                UX[i] += h0 * dt;
+                UY[i] += h1 * dt;
+                VHOX[i] = h0;
+                VHOY[i] = h1;
+                FOX[i] = fx;
+                FOY[i] = fy;
+            }
+            __syncthreads();
+            ++it;
+        }
+        if (tid == 0) {
+            iters_out[s] = it;
+            resid_out[s] = r;
+        }
+        __syncthreads();
+    }
+}
+
+} // namespace
+
+hipError_t launch_batch_adr(hipStream_t st, int grid, int block, const BatchView &B, double du, double tol,
+                            int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out)
+{
+    switch (block) {
+    case 256: k_batch_adr<256><<<grid, 256, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    case 512: k_batch_adr<512><<<grid, 512, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    case 1024: k_batch_adr<1024><<<grid, 1024, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}

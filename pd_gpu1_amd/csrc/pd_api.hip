@@ -138,6 +138,24 @@ int pick_group(const pd_handle *h)
     return 4;
 }
 
+// Constants of the mixed LPS passes.  With t = w (A theta xi + B e) / m, the
+// pair force (t_i + t_j) vol fac n of Force_State.cu:111-117 (2D: A =
+// (2nu-1)/(nu-1)(2K-32G/9), B = 8G) and Force_State.cu:274-279 (3D: 3K theta
+// w xi/m + 15G/m w (e - theta xi/3), i.e. A = 3K-5G, B = 15G).
+void lps_mixed_consts(const pd_params &P, float *ctheta, float *A, float *B)
+{
+    if (P.dim == 2) {
+        double c2 = (2.0 * P.pratio - 1.0) / (P.pratio - 1.0);
+        *ctheta = (float)(2.0 * c2);
+        *A = (float)(c2 * (2.0 * P.K - 32.0 * P.G / 9.0));
+        *B = (float)(8.0 * P.G);
+    } else {
+        *ctheta = 3.0f;
+        *A = (float)(3.0 * P.K - 5.0 * P.G);
+        *B = (float)(15.0 * P.G);
+    }
+}
+
 // Force evaluation (PMB, or LPS theta pass + force pass) on the handle's
 // stream.  stop may be null.
 int force_launch(pd_handle *h, const int *stop)
@@ -168,16 +186,7 @@ int force_launch(pd_handle *h, const int *stop)
         } else {
             const int g = pick_group(h);
             float ctheta, A, B;
-            if (P.dim == 2) {
-                double c2 = (2.0 * P.pratio - 1.0) / (P.pratio - 1.0);
-                ctheta = (float)(2.0 * c2);
-                A = (float)(c2 * (2.0 * P.K - 32.0 * P.G / 9.0));
-                B = (float)(8.0 * P.G);
-            } else {
-                ctheta = 3.0f;
-                A = (float)(3.0 * P.K - 5.0 * P.G);
-                B = (float)(15.0 * P.G);
-            }
+            lps_mixed_consts(P, &ctheta, &A, &B);
             LAUNCH(launch_lps_theta_mixed(s, stop, P.dim, g, h->N, h->row_ptr, h->col, h->xi32, h->coef32, h->fail,
                                           cd3(h->x), cd3(h->u), h->minv32, ctheta, P.theta_skip_broken,
                                           h->theta32));
@@ -692,6 +701,72 @@ int pd_force(pd_handle *h)
     int rc = force_launch(h, nullptr);
     if (rc) return rc;
     LAUNCH(hipEventRecord(h->ev1, h->stream));
+    return PD_OK;
+}
+
+int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
+{
+    // Average device time of each force pass over reps launches, timed with
+    // HIP events on the handle's stream: ms_out[0] = PMB kernel or LPS
+    // dilatation pass, ms_out[1] = LPS force pass (0 for PMB).
+    if (!h || !ms_out || reps <= 0) return PD_E_INVALID;
+    REQUIRE(h->col, "pd_build_neighbors first");
+    int rc = ensure_coef(h);
+    if (rc) return rc;
+    hipEvent_t e[3];
+    for (auto &ev : e) LAUNCH(hipEventCreate(&ev));
+    const pd_params &P = h->P;
+    const double ymin = P.dim == 3 ? P.break_y_min : -INFINITY;
+    double acc[2] = {0.0, 0.0};
+    for (int r = 0; r < reps && rc == PD_OK; ++r) {
+        LAUNCH(hipEventRecord(e[0], h->stream));
+        if (P.model == PD_MODEL_PMB) {
+            rc = force_launch(h, nullptr);
+            LAUNCH(hipEventRecord(e[1], h->stream));
+            LAUNCH(hipEventRecord(e[2], h->stream));
+        } else if (P.precision == PD_PREC_FP64) {
+            LAUNCH(launch_lps_theta_fp64(h->stream, nullptr, P.dim, h->N, h->row_ptr, h->col, cd3(h->x), cd3(h->u),
+                                         h->idist, h->fac, h->w, h->m, h->fail, P.vol, P.pratio,
+                                         P.theta_skip_broken, h->e, h->theta));
+            LAUNCH(hipEventRecord(e[1], h->stream));
+            LAUNCH(launch_lps_force_fp64(h->stream, nullptr, P.dim, h->N_int, h->row_ptr, h->col, cd3(h->x),
+                                         cd3(h->u), h->idist, h->fac, h->w, h->m, h->theta, h->e, P.vol, P.pratio,
+                                         P.K, P.G, P.s0, ymin, h->fail, h->f));
+            LAUNCH(hipEventRecord(e[2], h->stream));
+        } else {
+            const int g = pick_group(h);
+            float ctheta, A, B;
+            lps_mixed_consts(P, &ctheta, &A, &B);
+            LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, P.dim, g, h->N, h->row_ptr, h->col, h->xi32, h->coef32,
+                                          h->fail, cd3(h->x), cd3(h->u), h->minv32, ctheta, P.theta_skip_broken,
+                                          h->theta32));
+            LAUNCH(hipEventRecord(e[1], h->stream));
+            LAUNCH(launch_lps_force_mixed(h->stream, nullptr, P.dim, g, h->N_int, h->row_ptr, h->col, h->xi32,
+                                          h->coef32, h->fail, cd3(h->x), cd3(h->u), h->minv32, h->theta32, A, B, P.s0,
+                                          P.near_window, ymin, h->f32));
+            LAUNCH(hipEventRecord(e[2], h->stream));
+        }
+        LAUNCH(hipEventSynchronize(e[2]));
+        float t0 = 0.f, t1 = 0.f;
+        LAUNCH(hipEventElapsedTime(&t0, e[0], e[1]));
+        LAUNCH(hipEventElapsedTime(&t1, e[1], e[2]));
+        acc[0] += t0;
+        acc[1] += t1;
+    }
+    for (auto &ev : e) (void)hipEventDestroy(ev);
+    ms_out[0] = acc[0] / reps;
+    ms_out[1] = acc[1] / reps;
+    return rc;
+}
+
+int pd_set_group(pd_handle *h, int group)
+{
+    // Lanes per particle of the mixed kernels (0 = heuristic); the autotuner's knob.
+    if (!h) return PD_E_INVALID;
+    REQUIRE(group == 0 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
+            "group must be 0, 4, 8, 16, 32 or 64");
+    h->group = group;
+    drop_graph(h);
     return PD_OK;
 }
 

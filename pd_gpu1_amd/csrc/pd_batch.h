@@ -1,0 +1,25 @@
+// pd_batch.h -- device view of a batch of independent 2D PMB specimens.
+#pragma once
+#include "pd_internal.h"
+
+struct BatchSpec {
+    int n, n_int, n_top, n_bot;  // particles, integrated particles, strip counts
+    int64_t p_off;               // first particle in the concatenated arrays
+    int64_t rp_off;              // first row_ptr entry (each specimen stores n+1)
+    int64_t b_off;               // first bond; a multiple of 32 (fail words align)
+    int64_t nb;                  // bonds of this specimen
+};
+
+struct BatchView {
+    int nspec;
+    const BatchSpec *spec;
+    const double *x, *y;
+    double *ux, *uy, *fx, *fy, *vx, *vy, *vhx, *vhy, *vhox, *vhoy, *fox, *foy;
+    const int32_t *row_ptr, *col;
+    const double *kappa, *idist;
+    uint32_t *fail;
+    double mass, dt, s0;
+};
+
+hipError_t launch_batch_adr(hipStream_t st, int grid, int block, const BatchView &B, double du, double tol,
+                            int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out);

@@ -1,0 +1,313 @@
+// pd_batch.hip -- host side of the batched quasi-static ADR entry points.
+//
+// A batch concatenates independent square 2D PMB plates with boundary strips
+// (the Coord.cu:47-66 layout), each with its own centred, angled pre-crack
+// (Base_Function.cu:83-123) and local CSR bonds.  Every specimen is set up
+// with the single-problem pipeline of pd_api.hip (cell-list CSR, volume
+// correction, crack) and copied into the batch arrays; bond ranges start on
+// 32-bond boundaries so fail words copy as whole words.
+#include "pd_batch.h"
+
+#include <cstring>
+#include <vector>
+
+struct pd_batch {
+    pd_params P{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<BatchSpec> spec;
+    BatchSpec *d_spec = nullptr;
+    int64_t n_total = 0, nb_total = 0, rp_total = 0;
+    double *buf = nullptr;        // 14 particle fields of n_total doubles
+    int32_t *row_ptr = nullptr, *col = nullptr;
+    double *kappa = nullptr, *idist = nullptr;
+    uint32_t *fail = nullptr;
+    unsigned *queue = nullptr;
+    int32_t *d_iters = nullptr;
+    double *d_resid = nullptr;
+    int last_block = 1024;
+    BatchView view() const
+    {
+        BatchView v{};
+        v.nspec = (int)spec.size();
+        v.spec = d_spec;
+        double *f[14];
+        for (int k = 0; k < 14; k++) f[k] = buf + (size_t)k * n_total;
+        v.x = f[0]; v.y = f[1]; v.ux = f[2]; v.uy = f[3]; v.fx = f[4]; v.fy = f[5];
+        v.vx = f[6]; v.vy = f[7]; v.vhx = f[8]; v.vhy = f[9]; v.vhox = f[10]; v.vhoy = f[11];
+        v.fox = f[12]; v.foy = f[13];
+        v.row_ptr = row_ptr; v.col = col; v.kappa = kappa; v.idist = idist; v.fail = fail;
+        v.mass = P.mass; v.dt = P.dt; v.s0 = P.s0;
+        return v;
+    }
+};
+
+#define B_TRY(expr)                                                               \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            b->err = std::string(#expr) + ": " + hipGetErrorString(_e);           \
+            return PD_E_HIP;                                                      \
+        }                                                                         \
+    } while (0)
+
+static void batch_free(pd_batch *b)
+{
+    void *ptrs[] = {b->d_spec, b->buf, b->row_ptr, b->col, b->kappa, b->idist, b->fail, b->queue, b->d_iters,
+                    b->d_resid};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+extern "C" {
+
+int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int device, pd_batch **out)
+{
+    if (!p || !specs || nspec <= 0 || !out || p->dim != 2 || p->model != PD_MODEL_PMB) return PD_E_INVALID;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return PD_E_HIP;
+    pd_batch *b = new pd_batch();
+    b->P = *p;
+    b->P.precision = PD_PREC_FP64;
+    b->device = device;
+    auto fail_out = [&](int rc) {
+        batch_free(b);
+        delete b;
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail_out(PD_E_HIP);
+    // pass 1: build every specimen with the single-problem pipeline
+    std::vector<pd_handle *> hs(nspec, nullptr);
+    int rc = PD_OK;
+    for (int s = 0; s < nspec && rc == PD_OK; s++) {
+        const int nx = specs[s].nx, ny = specs[s].ny;
+        if (nx < 4 || ny < 4) {
+            rc = PD_E_INVALID;
+            break;
+        }
+        pd_params q = b->P;
+        q.n_top = q.n_bot = 3 * nx;
+        const int n = nx * (ny + 6);
+        std::vector<double> x(n), y(n);
+        pd_gen_plate_strips(nx, ny, q.dx, x.data(), y.data());
+        rc = pd_create(&q, device, &hs[s]);
+        if (!rc) rc = pd_set_positions(hs[s], x.data(), y.data(), nullptr, nx * ny, n);
+        if (!rc) rc = pd_build_neighbors(hs[s]);
+        if (!rc) rc = pd_init_state(hs[s], PD_INIT_STATIC);
+        if (!rc && specs[s].crack_len > 0.0)
+            rc = pd_set_crack_2d(hs[s], specs[s].crack_len, 0.0, 0.0, specs[s].crack_angle);
+        if (rc && hs[s]) b->err = std::string("specimen ") + std::to_string(s) + ": " + pd_last_error(hs[s]);
+        if (!rc) {
+            BatchSpec d{};
+            d.n = n;
+            d.n_int = nx * ny;
+            d.n_top = d.n_bot = 3 * nx;
+            d.p_off = b->n_total;
+            d.rp_off = b->rp_total;
+            d.b_off = b->nb_total;
+            d.nb = hs[s]->Nb;
+            b->n_total += n;
+            b->rp_total += n + 1;
+            b->nb_total += (d.nb + 31) / 32 * 32;
+            b->spec.push_back(d);
+        }
+    }
+    if (!rc && b->nb_total >= ((int64_t)1 << 31)) rc = PD_E_CAPACITY;
+    // pass 2: allocate the batch and copy the specimens in
+    auto alloc = [&](void **p, size_t bytes) {
+        return hipMalloc(p, bytes ? bytes : 16) == hipSuccess ? PD_OK : PD_E_HIP;
+    };
+    if (!rc) rc = alloc((void **)&b->buf, sizeof(double) * 14 * b->n_total);
+    if (!rc) rc = alloc((void **)&b->row_ptr, sizeof(int32_t) * b->rp_total);
+    if (!rc) rc = alloc((void **)&b->col, sizeof(int32_t) * b->nb_total);
+    if (!rc) rc = alloc((void **)&b->kappa, sizeof(double) * b->nb_total);
+    if (!rc) rc = alloc((void **)&b->idist, sizeof(double) * b->nb_total);
+    if (!rc) rc = alloc((void **)&b->fail, sizeof(uint32_t) * (b->nb_total / 32 + 1));
+    if (!rc) rc = alloc((void **)&b->d_spec, sizeof(BatchSpec) * nspec);
+    if (!rc) rc = alloc((void **)&b->queue, sizeof(unsigned));
+    if (!rc) rc = alloc((void **)&b->d_iters, sizeof(int32_t) * nspec);
+    if (!rc) rc = alloc((void **)&b->d_resid, sizeof(double) * nspec);
+    if (!rc && hipMemset(b->buf, 0, sizeof(double) * 14 * b->n_total) != hipSuccess) rc = PD_E_HIP;
+    for (int s = 0; s < nspec && !rc; s++) {
+        pd_handle *h = hs[s];
+        const BatchSpec &d = b->spec[s];
+        // make kappa current (vol*fac*bc*scr, scr = 1: no surface correction
+        // in the crack-growth family, Solve_Plate_Crack_Grow.cu:110-117)
+        pd_params q;
+        pd_get_params(h, &q);
+        q.precision = PD_PREC_FP64;
+        pd_set_params(h, &q);
+        if (pd_force(h) != PD_OK) {  // allocates kappa; the state is discarded below
+            rc = PD_E_HIP;
+            break;
+        }
+        (void)hipDeviceSynchronize();
+        std::vector<uint32_t> fw((size_t)((d.nb + 31) / 32));
+        if (!fw.empty()) pd_get_fail_bits(h, fw.data());
+        hipError_t e = hipSuccess;
+        auto cp = [&](void *dst, const void *src, size_t bytes) {
+            if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+        };
+        cp(b->buf + 0 * (size_t)b->n_total + d.p_off, h->x.p[0], sizeof(double) * d.n);
+        cp(b->buf + 1 * (size_t)b->n_total + d.p_off, h->x.p[1], sizeof(double) * d.n);
+        cp(b->row_ptr + d.rp_off, h->row_ptr, sizeof(int32_t) * (d.n + 1));
+        cp(b->col + d.b_off, h->col, sizeof(int32_t) * d.nb);
+        cp(b->kappa + d.b_off, h->kappa, sizeof(double) * d.nb);
+        cp(b->idist + d.b_off, h->idist, sizeof(double) * d.nb);
+        cp(b->fail + d.b_off / 32, fw.data(), sizeof(uint32_t) * fw.size());
+        if (e != hipSuccess) {
+            b->err = std::string("batch copy: ") + hipGetErrorString(e);
+            rc = PD_E_HIP;
+        }
+    }
+    if (!rc && hipMemcpy(b->d_spec, b->spec.data(), sizeof(BatchSpec) * nspec, hipMemcpyHostToDevice) != hipSuccess)
+        rc = PD_E_HIP;
+    for (pd_handle *h : hs)
+        if (h) pd_destroy(h);
+    if (rc) {
+        std::string msg = b->err;
+        int code = rc;
+        batch_free(b);
+        delete b;
+        (void)msg;
+        return code;
+    }
+    *out = b;
+    return PD_OK;
+}
+
+int pd_batch_destroy(pd_batch *b)
+{
+    if (!b) return PD_E_INVALID;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    delete b;
+    return PD_OK;
+}
+
+const char *pd_batch_last_error(pd_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int pd_batch_info(pd_batch *b, int64_t *n_particles, int64_t *n_bonds, int *nspec)
+{
+    if (!b) return PD_E_INVALID;
+    if (n_particles) *n_particles = b->n_total;
+    if (n_bonds) {
+        int64_t nb = 0;
+        for (const auto &d : b->spec) nb += d.nb;
+        *n_bonds = nb;
+    }
+    if (nspec) *nspec = (int)b->spec.size();
+    return PD_OK;
+}
+
+int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int block, int grid,
+                       int32_t *iters_out, double *resid_out, double *ms_out)
+{
+    // One load increment for every specimen: strips move by +du / -du, then
+    // ADR iterates each specimen until its residual <= tol (or max_iters).
+    if (!b || max_iters < 0 || !(tol >= 0.0)) return PD_E_INVALID;
+    if (block != 256 && block != 512 && block != 1024) {
+        b->err = "block must be 256, 512 or 1024";
+        return PD_E_INVALID;
+    }
+    B_TRY(hipSetDevice(b->device));
+    const int nspec = (int)b->spec.size();
+    if (grid <= 0) {
+        int dev_cu = 0;
+        B_TRY(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, b->device));
+        grid = dev_cu * (block == 1024 ? 2 : 4);
+    }
+    if (grid > nspec) grid = nspec;
+    hipEvent_t e0, e1;
+    B_TRY(hipEventCreate(&e0));
+    B_TRY(hipEventCreate(&e1));
+    B_TRY(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
+    B_TRY(hipEventRecord(e0, b->stream));
+    B_TRY(launch_batch_adr(b->stream, grid, block, b->view(), du, tol, max_iters, b->queue, b->d_iters, b->d_resid));
+    B_TRY(hipEventRecord(e1, b->stream));
+    B_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    B_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->last_block = block;
+    if (ms_out) *ms_out = ms;
+    if (iters_out) B_TRY(hipMemcpy(iters_out, b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
+    if (resid_out) B_TRY(hipMemcpy(resid_out, b->d_resid, sizeof(double) * nspec, hipMemcpyDeviceToHost));
+    return PD_OK;
+}
+
+static int batch_field(pd_batch *b, int field, int *slot)
+{
+    switch (field) {
+    case PD_FIELD_X0: *slot = 0; return PD_OK;
+    case PD_FIELD_U: *slot = 2; return PD_OK;
+    case PD_FIELD_F: *slot = 4; return PD_OK;
+    case PD_FIELD_V: *slot = 6; return PD_OK;
+    case PD_FIELD_VH: *slot = 8; return PD_OK;
+    case PD_FIELD_VHO: *slot = 10; return PD_OK;
+    case PD_FIELD_FO: *slot = 12; return PD_OK;
+    default: b->err = "unsupported batch field"; return PD_E_INVALID;
+    }
+}
+
+int pd_batch_get_field(pd_batch *b, int spec, int field, double *out)
+{
+    // [2][n] of specimen spec
+    if (!b || !out || spec < 0 || spec >= (int)b->spec.size()) return PD_E_INVALID;
+    int slot = 0;
+    int rc = batch_field(b, field, &slot);
+    if (rc) return rc;
+    const BatchSpec &d = b->spec[spec];
+    B_TRY(hipStreamSynchronize(b->stream));
+    for (int c = 0; c < 2; c++)
+        B_TRY(hipMemcpy(out + (size_t)c * d.n, b->buf + (size_t)(slot + c) * b->n_total + d.p_off,
+                        sizeof(double) * d.n, hipMemcpyDeviceToHost));
+    return PD_OK;
+}
+
+int pd_batch_set_field(pd_batch *b, int spec, int field, const double *in)
+{
+    if (!b || !in || spec < 0 || spec >= (int)b->spec.size()) return PD_E_INVALID;
+    int slot = 0;
+    int rc = batch_field(b, field, &slot);
+    if (rc) return rc;
+    if (slot == 0) {
+        b->err = "reference coordinates are fixed";
+        return PD_E_INVALID;
+    }
+    const BatchSpec &d = b->spec[spec];
+    B_TRY(hipStreamSynchronize(b->stream));
+    for (int c = 0; c < 2; c++)
+        B_TRY(hipMemcpy(b->buf + (size_t)(slot + c) * b->n_total + d.p_off, in + (size_t)c * d.n,
+                        sizeof(double) * d.n, hipMemcpyHostToDevice));
+    return PD_OK;
+}
+
+int pd_batch_get_csr(pd_batch *b, int spec, int32_t *row_ptr, int32_t *col, uint32_t *fail_words)
+{
+    if (!b || spec < 0 || spec >= (int)b->spec.size()) return PD_E_INVALID;
+    const BatchSpec &d = b->spec[spec];
+    B_TRY(hipStreamSynchronize(b->stream));
+    if (row_ptr) B_TRY(hipMemcpy(row_ptr, b->row_ptr + d.rp_off, sizeof(int32_t) * (d.n + 1), hipMemcpyDeviceToHost));
+    if (col) B_TRY(hipMemcpy(col, b->col + d.b_off, sizeof(int32_t) * d.nb, hipMemcpyDeviceToHost));
+    if (fail_words)
+        B_TRY(hipMemcpy(fail_words, b->fail + d.b_off / 32, sizeof(uint32_t) * ((d.nb + 31) / 32),
+                        hipMemcpyDeviceToHost));
+    return PD_OK;
+}
+
+int pd_batch_spec_info(pd_batch *b, int spec, int *n, int *n_int, int64_t *nb)
+{
+    if (!b || spec < 0 || spec >= (int)b->spec.size()) return PD_E_INVALID;
+    const BatchSpec &d = b->spec[spec];
+    if (n) *n = d.n;
+    if (n_int) *n_int = d.n_int;
+    if (nb) *nb = d.nb;
+    return PD_OK;
+}
+
+} // extern "C"

@@ -152,6 +152,15 @@ class Problem:
         self._c(lib().pd_last_force_ms(self._h, C.byref(ms)))
         return ms.value
 
+    def time_force_passes(self, reps=10):
+        """Average device ms of (PMB or LPS dilatation pass, LPS force pass)."""
+        out = np.zeros(2)
+        self._c(lib().pd_time_force_passes(self._h, reps, dptr(out)))
+        return float(out[0]), float(out[1])
+
+    def set_group(self, group):
+        self._c(lib().pd_set_group(self._h, group))
+
     def verlet_half1(self):
         self._c(lib().pd_verlet_half1(self._h))
 

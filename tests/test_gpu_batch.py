@@ -1,0 +1,71 @@
+"""Batched quasi-static ADR (persistent launch, device work queue) against the
+FP64 oracle run specimen by specimen with the same fixed reduction tree."""
+import numpy as np
+import pytest
+
+from pd_gpu1_amd import batch as PB
+from pd_gpu1_amd import scenarios
+from oracle import oracle as O
+from oracle.runner import OracleModel
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_specimen(p, n, clen, ang):
+    x, y = O.coord_plate_strips(n, n, p.dx)
+    sc = scenarios.params_dict(p)
+    sc["n_top"] = sc["n_bot"] = 3 * n
+    m = OracleModel(sc, x, y, None, n * n)
+    m.build_neighbors()
+    m.init_state()
+    if clen > 0:
+        m.set_crack_2d(clen, 0.0, 0.0, ang)
+    return m
+
+
+def oracle_increment(m, du, tol, max_iters, T):
+    Ni, nt = m.N_int, m.sc["n_top"]
+    m.U[1][Ni:Ni + nt] += du
+    m.U[1][Ni + nt:] += -du
+    sc = m.sc
+    it = 0
+    while True:
+        m.force()
+        cn, disp = O.adr_terms(2, Ni, sc["mass"], sc["dt"], m.U, m.F, m.FO, m.VHO)
+        rn, rd = O.resid_terms(2, Ni, m.N, m.F, m.B)
+        c = O.adr_c(O.sum_tree(cn, T), O.sum_tree(disp, T))
+        rnum, rden = O.sum_tree(rn, T), O.sum_tree(rd, T)
+        r = np.sqrt(rnum) / np.sqrt(rden) if rden > 0 else (np.inf if rnum > 0 else 0.0)
+        if r <= tol or it >= max_iters or r != r:
+            return it, r
+        O.adr_integrate(2, Ni, it, c, sc["mass"], sc["dt"], m.F, m.B, m.VH, m.V, m.VHO, m.U, m.FO)
+        it += 1
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+def test_batch_adr_bit_exact_against_oracle(block):
+    p = PB.plate_params()
+    specs = [(12, 12, 0.0025, 0.3), (17, 17, 0.003, -0.2), (9, 9, 0.0, 0.0), (21, 21, 0.004, 0.5),
+             (14, 14, 0.002, 0.0)]
+    ms = [oracle_specimen(p, n, cl, a) for n, _, cl, a in specs]
+    du, tol, max_it = 2.0e-5, 1e-6, 600
+    with PB.Batch(p, specs) as b:
+        for s, m in enumerate(ms):
+            rp, col, mask = b.csr(s)
+            assert np.array_equal(rp, m.row_ptr) and np.array_equal(col, m.col) and np.array_equal(mask, m.fail)
+        for inc in range(3):
+            its, res, _ = b.increment(du, tol, max_it, block=block, grid=3)
+            for s, m in enumerate(ms):
+                it_o, r_o = oracle_increment(m, du, tol, max_it, block)
+                assert its[s] == it_o and res[s] == r_o, (inc, s, its[s], it_o, res[s], r_o)
+                assert np.array_equal(b.get(s, "U"), m.stack("U")), (inc, s)
+                assert np.array_equal(b.csr(s)[2], m.fail), (inc, s)
+        assert all(r <= tol for r in res)
+
+
+def test_plate_family_is_seeded():
+    a = PB.plate_family(8)
+    b = PB.plate_family(8)
+    assert a == b
+    assert all(40 <= s[0] <= 200 for s in PB.plate_family(512))
+    assert all(abs(s[3]) <= np.radians(30) + 1e-12 for s in a)
