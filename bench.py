@@ -56,7 +56,7 @@ def load_traffic(kernel_key):
         return None
 
 
-def cpu_baseline(sample_dims=(200, 100, 50), evals=3):
+def cpu_baseline(sample_dims=(200, 200, 100), evals=8):
     """FP64 oracle (OpenMP) on a sub-block of the same lattice and physics."""
     import numpy as np
     from oracle.runner import OracleModel
@@ -94,8 +94,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--reps", type=int, default=20, help="launches per pass for the kernel timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = built-in choice)")
+    ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (HBM counter reference)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from pd_gpu1_amd.engine import stream_copy
+        gbs = stream_copy(args.copy_calibration, reps=5)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "reps": 5, "launches": 6, "GB_per_s": gbs}))
+        return
 
     if args.gpus != 1:
         print("bench.py: this engine runs one specimen on one GPU; multi-GPU runs are not part of "
@@ -109,8 +116,17 @@ def main():
     t_setup = time.perf_counter()
     sc = scenarios.c2_lps_block(precision=_lib.PD_PREC_MIXED)
     prob = sc.make(device=0)
+    prob.sync()
+    from pd_gpu1_amd import autotune
     if args.group:
-        prob.set_group(args.group)
+        groups, tuned = (args.group, args.group), {}
+        autotune.apply(prob, groups)
+    else:
+        groups, tuned = autotune.tune_group(prob)
+        # the tuning launches advanced the state: start the measured run from rest
+        prob.close()
+        prob = sc.make(device=0)
+        autotune.apply(prob, groups)
     prob.sync()
     t_setup = time.perf_counter() - t_setup
     N, N_int, nb = prob.N, prob.N_int, prob.num_bonds
@@ -135,7 +151,7 @@ def main():
 
     roofline = {
         "bound": "hbm",
-        "kernel": "k_lps_force_mixed<3,%d>" % (args.group or 64),
+        "kernel": "k_lps_force_mixed<3,%d>" % groups[1],
         "achieved": round(gbs_force, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -163,6 +179,8 @@ def main():
                                "mixed precision, 200 Verlet steps at dt=2e-8",
                    "particles": N, "bonds": nb, "parallelism": "single GPU",
                    "force_bond_updates_per_s": nb / ((ms_theta + ms_force) * 1e-3),
+                   "groups": list(groups),
+                   "autotune_ms": {str(k): [round(v[0], 3), round(v[1], 3)] for k, v in tuned.items()},
                    "broken_bonds_after_run": broken},
         "roofline": roofline,
     }

@@ -225,6 +225,28 @@ int pd_batch_get_field(pd_batch *b, int spec, int field, double *out);       /* 
 int pd_batch_set_field(pd_batch *b, int spec, int field, const double *in);
 int pd_batch_get_csr(pd_batch *b, int spec, int32_t *row_ptr, int32_t *col, uint32_t *fail_words);
 
+/* ---- learned ADR warm start (no reference counterpart) --------------------
+ * A message-passing network over the PD bond graph: 3 layers, hidden 64,
+ * SiLU, BF16 MFMA operands with FP32 accumulation, trained by its own
+ * forward/backward kernels with Adam.  Node features [N][8]: u (2), previous
+ * increment (2), damage, strip flags (2), 1; edge features [E][3]: reference
+ * bond vector / delta (2), intact flag.  Output [N][2]: the predicted
+ * displacement increment (scaled as the caller scaled the targets). */
+typedef struct pd_mpnn pd_mpnn;
+int pd_mpnn_create(int device, uint64_t seed, pd_mpnn **out);
+int pd_mpnn_destroy(pd_mpnn *m);
+const char *pd_mpnn_last_error(pd_mpnn *m);
+int pd_mpnn_num_params(pd_mpnn *m, int64_t *n);
+int pd_mpnn_get_params(pd_mpnn *m, float *out);
+int pd_mpnn_set_params(pd_mpnn *m, const float *in);   /* also resets Adam moments */
+/* Symmetric CSR (the bond list; rows ascending). */
+int pd_mpnn_set_graph(pd_mpnn *m, int n_nodes, const int32_t *row_ptr, int64_t n_edges, const int32_t *col);
+int pd_mpnn_set_features(pd_mpnn *m, const float *node_feat, const float *edge_feat, const float *target,
+                         const float *weight);
+int pd_mpnn_forward(pd_mpnn *m, float *y_out);
+int pd_mpnn_gradient(pd_mpnn *m, float *grad_out, float *loss_out);
+int pd_mpnn_train_step(pd_mpnn *m, float lr, float *loss_out);
+
 /* ---- timing and launch shape (no reference counterpart: the reference times
  *      whole runs with gettimeofday, Base_Function.cu:7-12) ---------------- */
 int pd_last_force_ms(pd_handle *h, double *ms);       /* events around the last pd_force */
@@ -234,6 +256,15 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out);
 /* Lanes per particle of the mixed kernels: 4, 8, 16, 32, 64, or 0 for the
  * built-in choice from the mean row length. */
 int pd_set_group(pd_handle *h, int group);
+/* The same for one pass: pass 0 = PMB or LPS dilatation, 1 = LPS force. */
+int pd_set_pass_group(pd_handle *h, int pass, int group);
+/* HBM reference: a plain device copy of `bytes` (multiple of 16) repeated
+ * reps times; *gbps = 2*bytes*reps/time.  Used to calibrate counters. */
+int pd_stream_copy(int device, int64_t bytes, int reps, double *gbps);
+/* Work order of the mixed kernels: particles grouped in cubic tiles of edge
+ * tile_cells*dx (default 16 in 3D, 64 in 2D; 0 = index order, -1 = default).
+ * Results do not depend on it; it sets which particles share an XCD's L2. */
+int pd_set_tile(pd_handle *h, int tile_cells);
 int pd_sync(pd_handle *h);
 
 #ifdef __cplusplus

@@ -29,9 +29,12 @@ EXPORTS = (
     "pd_adr_iterate", "pd_adr_solve", "pd_strip_displace", "pd_reduce_sum", "pd_damage",
     "pd_get_field", "pd_set_field", "pd_get_fail_bits", "pd_set_fail_bits", "pd_cylinder_init",
     "pd_cylinder_get", "pd_contact_step", "pd_save_text", "pd_checkpoint_write", "pd_checkpoint_read",
-    "pd_last_force_ms", "pd_time_force_passes", "pd_set_group", "pd_sync",
+    "pd_last_force_ms", "pd_time_force_passes", "pd_set_group", "pd_set_pass_group", "pd_stream_copy", "pd_set_tile", "pd_sync",
     "pd_batch_create", "pd_batch_destroy", "pd_batch_last_error", "pd_batch_info", "pd_batch_spec_info",
     "pd_batch_increment", "pd_batch_get_field", "pd_batch_set_field", "pd_batch_get_csr",
+    "pd_mpnn_create", "pd_mpnn_destroy", "pd_mpnn_last_error", "pd_mpnn_num_params", "pd_mpnn_get_params",
+    "pd_mpnn_set_params", "pd_mpnn_set_graph", "pd_mpnn_set_features", "pd_mpnn_forward",
+    "pd_mpnn_gradient", "pd_mpnn_train_step",
 )
 
 
@@ -60,6 +63,7 @@ _lib = None
 D = C.POINTER(C.c_double)
 I32 = C.POINTER(C.c_int32)
 U32 = C.POINTER(C.c_uint32)
+F32 = C.POINTER(C.c_float)
 H = C.c_void_p
 
 
@@ -114,6 +118,9 @@ def lib():
             "pd_last_force_ms": [H, D],
             "pd_time_force_passes": [H, C.c_int, D],
             "pd_set_group": [H, C.c_int],
+            "pd_set_tile": [H, C.c_int],
+            "pd_set_pass_group": [H, C.c_int, C.c_int],
+            "pd_stream_copy": [C.c_int, C.c_int64, C.c_int, D],
             "pd_sync": [H],
             "pd_batch_create": [C.POINTER(PdParams), C.POINTER(PdSpecimen), C.c_int, C.c_int, C.POINTER(H)],
             "pd_batch_destroy": [H],
@@ -124,6 +131,17 @@ def lib():
             "pd_batch_get_field": [H, C.c_int, C.c_int, D],
             "pd_batch_set_field": [H, C.c_int, C.c_int, D],
             "pd_batch_get_csr": [H, C.c_int, I32, I32, U32],
+            "pd_mpnn_create": [C.c_int, C.c_uint64, C.POINTER(H)],
+            "pd_mpnn_destroy": [H],
+            "pd_mpnn_last_error": [H],
+            "pd_mpnn_num_params": [H, C.POINTER(C.c_int64)],
+            "pd_mpnn_get_params": [H, F32],
+            "pd_mpnn_set_params": [H, F32],
+            "pd_mpnn_set_graph": [H, C.c_int, I32, C.c_int64, I32],
+            "pd_mpnn_set_features": [H, F32, F32, F32, F32],
+            "pd_mpnn_forward": [H, F32],
+            "pd_mpnn_gradient": [H, F32, F32],
+            "pd_mpnn_train_step": [H, C.c_float, F32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -131,6 +149,7 @@ def lib():
             fn.restype = C.c_int
         L.pd_last_error.restype = C.c_char_p
         L.pd_batch_last_error.restype = C.c_char_p
+        L.pd_mpnn_last_error.restype = C.c_char_p
         _lib = L
     return _lib
 

@@ -1,0 +1,402 @@
+// k_force_mixed.hip -- mixed-precision PMB and LPS force kernels (throughput
+// path) and the particle-record packing they gather from.
+//
+// Layout.  Neighbour state is gathered from a 64-byte record per particle:
+// doubles [0..2] reference position x0, [3] two floats (theta/m, 1/m) written
+// by the LPS dilatation pass, [4..6] displacement u, [7] zero.  One gather
+// reads one 64-byte half line instead of six scattered SoA words.  The u half
+// is a whole 32-byte sector, rewritten by the integrators that update u (or by
+// the pack kernel when u changed elsewhere); the SoA arrays stay the state of
+// record.
+//
+// Work split.  A group of G lanes (G in 4..64, a divisor of the wave64) owns
+// one particle; lanes stride over its CSR row two bonds at a time so that
+// col, xi, the bond coefficient and the fail word of both bonds, and both
+// neighbour records, are in flight together.  Blocks are renumbered so that
+// each of the 8 XCDs (which receive workgroups round-robin) walks one
+// contiguous eighth of the particle range, keeping neighbour records in that
+// XCD's L2.
+//
+// Arithmetic.  The stretch is formed without cancellation from
+// eta = u_j - u_i and xi = x0_j - x0_i (both differenced in FP64, then FP32):
+//     s = eta.(2 xi + eta) / (|xi| (|xi + eta| + |xi|)),
+// equal to (|xi+eta| - |xi|)/|xi| of Force_Bond.cu:189.  Per-lane sums use
+// Neumaier compensation; the group sum combines lanes with error-free TwoSum
+// steps over xor shuffles.  A bond whose FP32 stretch lies within
+// near_window*s0 of s0 is re-evaluated with the FP64 parity expression
+// (Force_Bond.cu:185-189, Surf_Corr.cu:324-326) before its break is decided,
+// so breaks follow the FP64 stretch of the current state.
+#include "pd_internal.h"
+
+namespace {
+
+constexpr int kRec = 8;  // doubles per particle record
+
+__device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v); }
+__device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
+// Workgroups are handed to the 8 XCDs round-robin; give XCD k the contiguous
+// block range [k*per, (k+1)*per).  gridDim.x is a multiple of 8.
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = (int)(gridDim.x >> 3);
+    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+}
+
+struct Rec {
+    double x0, x1, x2, u0, u1, u2;
+};
+
+struct RecQ {
+    Rec r;
+    float q, m;  // theta/m and 1/m
+};
+
+__device__ __forceinline__ RecQ load_rec(const double *__restrict__ rec, int c)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(rec + (size_t)c * kRec);
+    const double2 a = p[0], b = p[1], d = p[2], e = p[3];
+    const float2 qr = __builtin_bit_cast(float2, b.y);
+    return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
+}
+
+__device__ __forceinline__ bool bit_of(uint32_t word, int32_t k) { return (word >> (k & 31)) & 1u; }
+
+__device__ __forceinline__ void bond_break(uint32_t *fail, int32_t k)
+{
+    atomicAnd(&fail[k >> 5], ~(1u << (k & 31)));
+}
+
+// Per-lane sums accumulate the FP32 bond terms in FP64: every FP32 term is
+// exact in FP64, so the running sum carries 29 guard bits (a compensated sum
+// tighter than Neumaier's in FP32).  The group sum reduces those FP64
+// partials over xor shuffles and rounds once to FP32.
+template <int G>
+__device__ __forceinline__ float group_sum(double a)
+{
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    return (float)a;
+}
+
+// Geometry of one bond in FP32 from FP64 differences.
+template <int DIM>
+struct BondGeo {
+    float dx, dy, dz;   // xi + eta
+    float num;          // |xi+eta|^2 - |xi|^2 = eta.(2 xi + eta)
+    float rL;           // 1/|xi+eta|
+    float L;            // |xi+eta|
+};
+
+template <int DIM>
+__device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
+{
+    const float ex = (float)(rc.x0 - ri.x0), ey = (float)(rc.x1 - ri.x1);
+    const float hx = (float)(rc.u0 - ri.u0), hy = (float)(rc.u1 - ri.u1);
+    BondGeo<DIM> g;
+    g.dx = ex + hx;
+    g.dy = ey + hy;
+    float L2 = g.dx * g.dx + g.dy * g.dy;
+    g.num = hx * (2.f * ex + hx) + hy * (2.f * ey + hy);
+    if (DIM == 3) {
+        const float ez = (float)(rc.x2 - ri.x2), hz = (float)(rc.u2 - ri.u2);
+        g.dz = ez + hz;
+        L2 += g.dz * g.dz;
+        g.num += hz * (2.f * ez + hz);
+    } else {
+        g.dz = 0.f;
+    }
+    g.rL = frsq(L2);
+    g.L = L2 * g.rL;
+    return g;
+}
+
+// FP64 stretch with the parity expressions (PMB order (x_c+u_c)-x_i-u_i for
+// every component, LPS-3D order for z when lps3d).
+template <int DIM, bool LPS3Z>
+__device__ __forceinline__ double stretch64(const Rec &ri, const Rec &rc)
+{
+    double d0 = rc.x0 + rc.u0 - ri.x0 - ri.u0, d1 = rc.x1 + rc.u1 - ri.x1 - ri.u1;
+    double a0 = rc.x0 - ri.x0, a1 = rc.x1 - ri.x1;
+    double r2 = d0 * d0 + d1 * d1, q2 = a0 * a0 + a1 * a1;
+    if (DIM == 3) {
+        double d2 = LPS3Z ? rc.x2 - ri.x2 + rc.u2 - ri.u2 : rc.x2 + rc.u2 - ri.x2 - ri.u2;
+        double a2 = rc.x2 - ri.x2;
+        r2 = r2 + d2 * d2;
+        q2 = q2 + a2 * a2;
+    }
+    double id = sqrt(q2);
+    return (sqrt(r2) - id) / id;
+}
+
+template <int DIM>
+__device__ __forceinline__ void pmb_bond(const Rec &ri, const Rec &rc, float L0, float kap, bool ok, int32_t k,
+                                         float s0f, float win, double s0, double ymin, double &ax, double &ay,
+                                         double &az, uint32_t *fail)
+{
+    const BondGeo<DIM> g = geo<DIM>(ri, rc);
+    const float s = g.num * frcp(L0 * (g.L + L0));
+    const float t = ok ? kap * s * g.rL : 0.f;
+    ax += (double)(t * g.dx);
+    ay += (double)(t * g.dy);
+    if (DIM == 3) az += (double)(t * g.dz);
+    if (ok && s >= s0f - win) {
+        const bool brk = (s > s0f + win ? true : stretch64<DIM, false>(ri, rc) > s0) &&
+                         (DIM == 2 || (ri.x1 >= ymin && rc.x1 >= ymin));
+        if (brk) bond_break(fail, k);
+    }
+}
+
+template <int DIM, int G>
+__global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop, const int32_t *__restrict__ order, int n_rows,
+                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                   const float2 *__restrict__ bnd, uint32_t *fail, const double *__restrict__ rec, F3 f, double s0,
+                                                   float s0f, float win, double ymin)
+{
+    if (stop && *stop) return;
+    const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (gid >= n_rows) return;  // the whole group leaves together
+    const int i = order ? order[gid] : gid;
+    const Rec ri = load_rec(rec, i).r;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int32_t kb = rp[i], ke = rp[i + 1];
+    for (int32_t k = kb + lane; k < ke; k += 2 * G) {
+        const bool two = k + G < ke;
+        const int32_t k2 = two ? k + G : k;
+        const int c1 = col[k], c2 = col[k2];
+        const float2 b1 = bnd[k], b2 = bnd[k2];
+        const uint32_t w1 = fail[k >> 5], w2 = fail[k2 >> 5];
+        const Rec r1 = load_rec(rec, c1).r, r2 = load_rec(rec, c2).r;
+        pmb_bond<DIM>(ri, r1, b1.x, b1.y, bit_of(w1, k), k, s0f, win, s0, ymin, ax, ay, az, fail);
+        if (two) pmb_bond<DIM>(ri, r2, b2.x, b2.y, bit_of(w2, k2), k2, s0f, win, s0, ymin, ax, ay, az, fail);
+    }
+    const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
+    const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
+    if (lane == 0) {
+        f.p[0][i] = sx;
+        f.p[1][i] = sy;
+        if (DIM == 3) f.p[2][i] = sz;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void theta_bond(const Rec &ri, const Rec &rc, float L0, float om, bool ok, double &acc)
+{
+    const BondGeo<DIM> g = geo<DIM>(ri, rc);
+    const float e = g.num * frcp(g.L + L0);  // |xi+eta| - |xi|
+    acc += (double)(ok ? om * L0 * e : 0.f);
+}
+
+template <int DIM, int G>
+__global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__ stop, const int32_t *__restrict__ order, int N,
+                                                         const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ col,
+                                                         const float2 *__restrict__ bnd,
+                                                         const uint32_t *__restrict__ fail, double *rec,
+                                                         const float *__restrict__ minv32, float ctheta,
+                                                         int skip_broken, float *theta32)
+{
+    if (stop && *stop) return;
+    const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (gid >= N) return;
+    const int i = order ? order[gid] : gid;
+    const Rec ri = load_rec(rec, i).r;
+    double acc = 0.0;
+    const int32_t kb = rp[i], ke = rp[i + 1];
+    for (int32_t k = kb + lane; k < ke; k += 2 * G) {
+        const bool two = k + G < ke;
+        const int32_t k2 = two ? k + G : k;
+        const int c1 = col[k], c2 = col[k2];
+        const float2 b1 = bnd[k], b2 = bnd[k2];
+        const uint32_t w1 = fail[k >> 5], w2 = fail[k2 >> 5];
+        const Rec r1 = load_rec(rec, c1).r, r2 = load_rec(rec, c2).r;
+        theta_bond<DIM>(ri, r1, b1.x, b1.y, !skip_broken || bit_of(w1, k), acc);
+        if (two) theta_bond<DIM>(ri, r2, b2.x, b2.y, !skip_broken || bit_of(w2, k2), acc);
+    }
+    const float th = group_sum<G>(acc);
+    if (lane == 0) {
+        const float r = minv32[i];
+        const float theta = ctheta * r * th;
+        theta32[i] = theta;
+        *reinterpret_cast<float2 *>(rec + (size_t)i * kRec + 3) = make_float2(theta * r, r);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void lps_bond(const Rec &ri, const RecQ &rc, float qi, float ri_m, float L0, float om,
+                                         bool ok, int32_t k, float A, float B, float s0f, float win, double s0,
+                                         double ymin, double &ax, double &ay, double &az, uint32_t *fail)
+{
+    const BondGeo<DIM> g = geo<DIM>(ri, rc.r);
+    const float e = g.num * frcp(g.L + L0);
+    const float s = e * frcp(L0);
+    // (t_i + t_j) vol fac = omega [A xi (q_i + q_j) + B e (r_i + r_j)]
+    const float tt = ok ? om * (A * L0 * (qi + rc.q) + B * e * (ri_m + rc.m)) * g.rL : 0.f;
+    ax += (double)(tt * g.dx);
+    ay += (double)(tt * g.dy);
+    if (DIM == 3) az += (double)(tt * g.dz);
+    if (ok && s >= s0f - win) {
+        const bool brk = (s > s0f + win ? true : stretch64<DIM, true>(ri, rc.r) > s0) &&
+                         (DIM == 2 || (ri.x1 >= ymin && rc.r.x1 >= ymin));
+        if (brk) bond_break(fail, k);
+    }
+}
+
+template <int DIM, int G>
+__global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__ stop, const int32_t *__restrict__ order, int N_int,
+                                                         const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ col,
+                                                         const float2 *__restrict__ bnd, uint32_t *fail,
+                                                         const double *__restrict__ rec, float A, float B,
+                                                         double s0, float s0f, float win, double ymin, F3 f)
+{
+    if (stop && *stop) return;
+    const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (gid >= N_int) return;
+    const int i = order ? order[gid] : gid;
+    const RecQ qi = load_rec(rec, i);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int32_t kb = rp[i], ke = rp[i + 1];
+    for (int32_t k = kb + lane; k < ke; k += 2 * G) {
+        const bool two = k + G < ke;
+        const int32_t k2 = two ? k + G : k;
+        const int c1 = col[k], c2 = col[k2];
+        const float2 b1 = bnd[k], b2 = bnd[k2];
+        const uint32_t w1 = fail[k >> 5], w2 = fail[k2 >> 5];
+        const RecQ r1 = load_rec(rec, c1), r2 = load_rec(rec, c2);
+        lps_bond<DIM>(qi.r, r1, qi.q, qi.m, b1.x, b1.y, bit_of(w1, k), k, A, B, s0f, win, s0, ymin, ax, ay, az, fail);
+        if (two)
+            lps_bond<DIM>(qi.r, r2, qi.q, qi.m, b2.x, b2.y, bit_of(w2, k2), k2, A, B, s0f, win, s0, ymin, ax, ay, az,
+                          fail);
+    }
+    const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
+    const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
+    if (lane == 0) {
+        f.p[0][i] = sx;
+        f.p[1][i] = sy;
+        if (DIM == 3) f.p[2][i] = sz;
+    }
+}
+
+__global__ void k_pack_x0(int dim, int N, CD3 x, double *rec)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double *r = rec + (size_t)i * kRec;
+    r[0] = x.p[0][i];
+    r[1] = x.p[1][i];
+    r[2] = dim == 3 ? x.p[2][i] : 0.0;
+    r[3] = 0.0;
+}
+
+__global__ void k_pack_u(const int *__restrict__ stop, int dim, int N, CD3 u, double *rec)
+{
+    if (stop && *stop) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double2 *r = reinterpret_cast<double2 *>(rec + (size_t)i * kRec + 4);
+    r[0] = make_double2(u.p[0][i], u.p[1][i]);
+    r[1] = make_double2(dim == 3 ? u.p[2][i] : 0.0, 0.0);
+}
+
+inline int grid8(int64_t threads)
+{
+    int64_t b = (threads + 255) / 256;
+    return (int)((b + 7) / 8 * 8);
+}
+
+template <int DIM, int G>
+hipError_t pmb_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
+                 const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window, double ymin)
+{
+    k_pmb_mixed<DIM, G><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, f, s0,
+                                                              (float)s0, (float)(window * s0), ymin);
+    return hipGetLastError();
+}
+
+template <int DIM, int G>
+hipError_t theta_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
+                   const float2 *bnd, const uint32_t *fail, double *rec, const float *minv32, float ctheta, int skip,
+                   float *theta32)
+{
+    k_lps_theta_mixed<DIM, G><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, minv32,
+                                                                    ctheta, skip, theta32);
+    return hipGetLastError();
+}
+
+template <int DIM, int G>
+hipError_t lpsf_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
+                  const float2 *bnd, uint32_t *fail, const double *rec, float A, float B, double s0, double window,
+                  double ymin, F3 f)
+{
+    k_lps_force_mixed<DIM, G><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, A, B,
+                                                                    s0, (float)s0, (float)(window * s0), ymin, f);
+    return hipGetLastError();
+}
+
+#define PD_DISPATCH_G(FN, DIM, G, ...)          \
+    switch (G) {                                \
+    case 4: return FN<DIM, 4>(__VA_ARGS__);     \
+    case 8: return FN<DIM, 8>(__VA_ARGS__);     \
+    case 16: return FN<DIM, 16>(__VA_ARGS__);   \
+    case 32: return FN<DIM, 32>(__VA_ARGS__);   \
+    case 64: return FN<DIM, 64>(__VA_ARGS__);   \
+    default: return hipErrorInvalidValue;       \
+    }
+
+} // namespace
+
+hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double *rec)
+{
+    if (N <= 0) return hipSuccess;
+    k_pack_x0<<<(N + 255) / 256, 256, 0, s>>>(dim, N, x, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, double *rec)
+{
+    if (N <= 0) return hipSuccess;
+    k_pack_u<<<(N + 255) / 256, 256, 0, s>>>(stop, dim, N, u, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
+                            const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
+                            const double *rec, F3 f, double s0, double window, double ymin)
+{
+    if (n_rows <= 0) return hipSuccess;
+    if (dim == 3) {
+        PD_DISPATCH_G(pmb_g, 3, group, s, stop, order, n_rows, rp, col, bnd, fail, rec, f, s0, window, ymin)
+    } else {
+        PD_DISPATCH_G(pmb_g, 2, group, s, stop, order, n_rows, rp, col, bnd, fail, rec, f, s0, window, ymin)
+    }
+}
+
+hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
+                                  const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
+                                  double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32)
+{
+    if (N <= 0) return hipSuccess;
+    if (dim == 3) {
+        PD_DISPATCH_G(theta_g, 3, group, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, skip_broken,
+                      theta32)
+    } else {
+        PD_DISPATCH_G(theta_g, 2, group, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, skip_broken,
+                      theta32)
+    }
+}
+
+hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
+                                  const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
+                                  const double *rec, float A, float B, double s0, double window, double ymin, F3 f)
+{
+    if (N_int <= 0) return hipSuccess;
+    if (dim == 3) {
+        PD_DISPATCH_G(lpsf_g, 3, group, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, window, ymin, f)
+    } else {
+        PD_DISPATCH_G(lpsf_g, 2, group, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, window, ymin, f)
+    }
+}

@@ -16,6 +16,17 @@
 
 namespace {
 
+// Mixed-precision force kernels gather u from the displacement half of a
+// 64-byte particle record (doubles 4..7: u_x, u_y, u_z, 0).  Kernels that move
+// particles refresh that half in the same pass when given the record array.
+__device__ __forceinline__ void store_rec_u(double *rec, int i, double ux, double uy, double uz)
+{
+    double2 *r = reinterpret_cast<double2 *>(rec + (size_t)i * 8 + 4);
+    r[0] = make_double2(ux, uy);
+    r[1] = make_double2(uz, 0.0);
+}
+
+
 template <int NC>
 struct Vec {
     double v[NC];
@@ -77,7 +88,7 @@ __device__ __forceinline__ bool grid_canon(Vec<NC> &p, double *partials, DevScal
 }
 
 __global__ void k_verlet1(int dim, int N_int, int n_top, int n_bot, double dt, double load, D3 v, CD3 a,
-                          D3 u, double *cyl)
+                          D3 u, double *cyl, double *rec)
 {
     // Integrate.cu:447-463 (i < N_int) and Solve_Plate_Crack_Grow.cu:20-27
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,6 +102,8 @@ __global__ void k_verlet1(int dim, int N_int, int n_top, int n_bot, double dt, d
     } else if (i < N_int + n_top + n_bot) {
         u.p[1][i] -= load * dt;
     }
+    if (rec && i < N_int + n_top + n_bot)
+        store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
     if (cyl && i == 0) {
         // Solve_Kalthoff_Winkler.cu:101-110: vel += dt/2*acc; disp += vel*dt
         for (int d = 0; d < 3; ++d) {
@@ -185,7 +198,7 @@ __global__ __launch_bounds__(1024) void k_adr_reduce(int dim, int N_int, int N, 
 }
 
 __global__ void k_adr_update(int dim, int N_int, double mass, double dt, CD3 f, CF3 f32, int use32, CD3 b,
-                             D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S)
+                             D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S, double *rec)
 {
     // kernel_integrate, Integrate.cu:62-105
     if (S->done) return;
@@ -206,6 +219,7 @@ __global__ void k_adr_update(int dim, int N_int, double mass, double dt, CD3 f, 
         vho.p[d][i] = hv;
         fo.p[d][i] = fd;
     }
+    if (rec) store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
 }
 
 __global__ void k_set_ct(DevScalars *S, int ct, int clear_done)
@@ -219,7 +233,7 @@ __global__ void k_set_ct(DevScalars *S, int ct, int clear_done)
 
 __global__ __launch_bounds__(1024) void k_contact(int N, const double *__restrict__ dmg, CD3 x, D3 u, D3 v,
                                                   pd_cyl cyl, double dt, double dens, double vol,
-                                                  double *partials, DevScalars *S)
+                                                  double *partials, DevScalars *S, double *rec)
 {
     // Solve_Kalthoff_Winkler.cu:177-239; the reaction is summed in canonical
     // order instead of by float atomicAdd (Solve_Kalthoff_Winkler.cu:168-170),
@@ -261,6 +275,7 @@ __global__ __launch_bounds__(1024) void k_contact(int N, const double *__restric
                 u.p[0][i] = nux;
                 u.p[1][i] = nuy;
                 u.p[2][i] = nuz;
+                if (rec) store_rec_u(rec, i, nux, nuy, nuz);
                 double nvx = (nux - oux) / dt, nvy = (nuy - ouy) / dt, nvz = (nuz - ouz) / dt;
                 v.p[0][i] = nvx;
                 v.p[1][i] = nvy;
@@ -319,11 +334,11 @@ inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 } // namespace
 
 hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
-                          D3 v, CD3 a, D3 u, double *cyl_state)
+                          D3 v, CD3 a, D3 u, double *cyl_state, double *rec)
 {
     int n = N_int + n_top + n_bot;
     if (n < 1) n = 1;
-    k_verlet1<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, load, v, a, u, cyl_state);
+    k_verlet1<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, load, v, a, u, cyl_state, rec);
     return hipGetLastError();
 }
 
@@ -345,10 +360,11 @@ hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, in
 }
 
 hipError_t launch_adr_update(hipStream_t s, int dim, int N_int, double mass, double dt, CD3 f, CF3 f32,
-                             int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S)
+                             int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S, double *rec)
 {
     if (N_int <= 0) return hipSuccess;
-    k_adr_update<<<nblk(N_int, 256), 256, 0, s>>>(dim, N_int, mass, dt, f, f32, use32, b, vh, v, vho, u, fo, S);
+    k_adr_update<<<nblk(N_int, 256), 256, 0, s>>>(dim, N_int, mass, dt, f, f32, use32, b, vh, v, vho, u, fo, S,
+                                                  rec);
     return hipGetLastError();
 }
 
@@ -359,9 +375,10 @@ hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done)
 }
 
 hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u, D3 v,
-                          pd_cyl cyl, double dt, double dens, double vol, double *partials, DevScalars *S)
+                          pd_cyl cyl, double dt, double dens, double vol, double *partials, DevScalars *S,
+                          double *rec)
 {
-    k_contact<<<blocks, threads, 0, s>>>(N, dmg, x, u, v, cyl, dt, dens, vol, partials, S);
+    k_contact<<<blocks, threads, 0, s>>>(N, dmg, x, u, v, cyl, dt, dens, vol, partials, S, rec);
     return hipGetLastError();
 }
 

@@ -94,6 +94,20 @@ __global__ void k_neigh(int N, CD3 x, Grid g, const int32_t *__restrict__ cell_s
     if (!FILL) cnt[i] = n;
 }
 
+__global__ void k_tile_keys(int dim, int N, int n_int, CD3 x, Grid g, uint32_t *key, uint32_t *idx)
+{
+    // tile id of each particle, tiles numbered x-fastest then y then z;
+    // particles i >= n_int sort after every integrated particle
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c0 = cell_coord(x.p[0][i], g.lo[0], g.cs, g.n[0]);
+    int c1 = cell_coord(x.p[1][i], g.lo[1], g.cs, g.n[1]);
+    int c2 = dim == 3 ? cell_coord(x.p[2][i], g.lo[2], g.cs, g.n[2]) : 0;
+    uint32_t t = (uint32_t)((c2 * g.n[1] + c1) * g.n[0] + c0);
+    key[i] = i < n_int ? t : t + (uint32_t)(g.n[0] * g.n[1] * g.n[2]);
+    idx[i] = (uint32_t)i;
+}
+
 // One 64-lane wave sorts one row in LDS; four rows per 256-thread block.
 template <int NP2>
 __global__ __launch_bounds__(256) void k_row_sort(int N, const int32_t *__restrict__ row_ptr,
@@ -269,6 +283,45 @@ int build_neighbors_impl(pd_handle *h)
     NB_TRY(launch_neigh_fill(s, dim, N, x, h->lo, cs, h->ncell, cell_start.as<int32_t>(), items.as<uint32_t>(), d2,
                              h->row_ptr, h->col));
     NB_TRY(launch_row_sort(s, N, h->row_ptr, h->col, max_row > 0 ? max_row : 1));
+    NB_TRY(hipStreamSynchronize(s));
+    return PD_OK;
+}
+
+int build_tile_order(pd_handle *h, double tile_len)
+{
+    // Processing order of the throughput kernels: particles grouped by cubic
+    // tiles of edge tile_len (stable inside a tile), so the particles that run
+    // together on one XCD gather their neighbours from one compact region.
+    // Integrated particles (i < N_int) come first, so order[0, N_int) is the
+    // work list of the passes that only visit integrated rows.
+    const int N = h->N, dim = h->P.dim;
+    hipStream_t s = h->stream;
+    Grid g{{h->lo[0], h->lo[1], h->lo[2]}, tile_len, {1, 1, 1}};
+    double hi[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) {
+        hi[d] = h->lo[d] + h->ncell[d] * h->P.delta * (1.0 + 1e-9);
+        g.n[d] = (int)std::floor((hi[d] - h->lo[d]) / tile_len) + 1;
+    }
+    if ((int64_t)g.n[0] * g.n[1] * g.n[2] >= ((int64_t)1 << 30)) {
+        h->err = "too many tiles";
+        return PD_E_CAPACITY;
+    }
+    int ntiles = 2 * g.n[0] * g.n[1] * g.n[2];
+    int end_bit = 1;
+    while (end_bit < 32 && ((int64_t)1 << end_bit) <= ntiles) ++end_bit;
+    DevBuf key, idx, key2, tmp;
+    NB_TRY(key.alloc(sizeof(uint32_t) * N));
+    NB_TRY(idx.alloc(sizeof(uint32_t) * N));
+    NB_TRY(key2.alloc(sizeof(uint32_t) * N));
+    if (!h->order) NB_TRY(hipMalloc(&h->order, sizeof(int32_t) * (N > 0 ? N : 1)));
+    k_tile_keys<<<(N + 255) / 256, 256, 0, s>>>(dim, N, h->N_int, cd3(h->x), g, key.as<uint32_t>(), idx.as<uint32_t>());
+    NB_TRY(hipGetLastError());
+    size_t tb = 0;
+    NB_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.as<uint32_t>(), key2.as<uint32_t>(), idx.as<uint32_t>(),
+                                              reinterpret_cast<uint32_t *>(h->order), N, 0, end_bit, s));
+    NB_TRY(tmp.alloc(tb));
+    NB_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.as<uint32_t>(), key2.as<uint32_t>(), idx.as<uint32_t>(),
+                                              reinterpret_cast<uint32_t *>(h->order), N, 0, end_bit, s));
     NB_TRY(hipStreamSynchronize(s));
     return PD_OK;
 }

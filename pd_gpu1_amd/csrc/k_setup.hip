@@ -200,7 +200,7 @@ __global__ void k_lps_weights(int N, const int32_t *__restrict__ rp, const doubl
 
 __global__ void k_pmb_coef(int64_t nb, const double *__restrict__ fac, const double *__restrict__ scr,
                            double vol, double bc, const double *__restrict__ idist, double *kappa,
-                           float *xi32, float *coef32)
+                           float2 *bnd)
 {
     // kappa = vol*fac*bc*scr: the left prefix of Force_Bond.cu:190, so
     // kappa*s*d/nl reproduces the reference product bit for bit.
@@ -208,19 +208,17 @@ __global__ void k_pmb_coef(int64_t nb, const double *__restrict__ fac, const dou
     if (k >= nb) return;
     double kap = vol * fac[k] * bc * scr[k];
     kappa[k] = kap;
-    xi32[k] = (float)idist[k];
-    coef32[k] = (float)kap;
+    bnd[k] = make_float2((float)idist[k], (float)kap);
 }
 
 __global__ void k_lps_coef(int64_t nb, const double *__restrict__ w, const double *__restrict__ fac,
-                           double vol, const double *__restrict__ idist, float *xi32, float *om32)
+                           double vol, const double *__restrict__ idist, float2 *bnd)
 {
     // omega = w*fac*vol per bond, the bond factor shared by the dilatation
     // and force sums of Force_State.cu:224,274-279.
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
-    om32[k] = (float)(w[k] * fac[k] * vol);
-    xi32[k] = (float)idist[k];
+    bnd[k] = make_float2((float)idist[k], (float)(w[k] * fac[k] * vol));
 }
 
 __global__ void k_inv_to_f32(const double *a, float *b, int n)
@@ -244,9 +242,22 @@ __global__ void k_damage(int N_int, const int32_t *__restrict__ rp, const uint32
     dmg[i] = 1.0 - p1 / p2;
 }
 
+__global__ void k_copy16(const double2 *__restrict__ a, double2 *__restrict__ b, int64_t n)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; t < n; t += (int64_t)gridDim.x * blockDim.x) b[t] = a[t];
+}
+
 inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
 } // namespace
+
+hipError_t launch_copy16(hipStream_t s, const void *a, void *b, int64_t bytes)
+{
+    int64_t n = bytes / 16;
+    k_copy16<<<8192, 256, 0, s>>>((const double2 *)a, (double2 *)b, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_vol_corr(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col, CD3 x,
                            double delta, double dx, double *idist, double *fac)
@@ -310,18 +321,17 @@ hipError_t launch_lps_weights(hipStream_t s, int N, const int32_t *rp, const dou
 }
 
 hipError_t launch_pmb_coef(hipStream_t s, int64_t nb, const double *fac, const double *scr, double vol,
-                           double bc, const double *idist, double *kappa, float *xi32, float *coef32)
+                           double bc, const double *idist, double *kappa, float2 *bnd)
 {
     if (nb <= 0) return hipSuccess;
-    k_pmb_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, fac, scr, vol, bc, idist, kappa, xi32, coef32);
+    k_pmb_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, fac, scr, vol, bc, idist, kappa, bnd);
     return hipGetLastError();
 }
 
 hipError_t launch_lps_coef(hipStream_t s, int64_t nb, const double *w, const double *fac, double vol,
-                           const double *idist, float *xi32, float *coef32, int N, const double *m,
-                           float *m32)
+                           const double *idist, float2 *bnd, int N, const double *m, float *m32)
 {
-    if (nb > 0) k_lps_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, w, fac, vol, idist, xi32, coef32);
+    if (nb > 0) k_lps_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, w, fac, vol, idist, bnd);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_inv_to_f32<<<nblk(N, 256), 256, 0, s>>>(m, m32, N);

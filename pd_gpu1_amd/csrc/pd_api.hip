@@ -59,8 +59,7 @@ void free_bonds(pd_handle *h)
     dfree(h->kappa);
     dfree(h->w);
     dfree(h->e);
-    dfree(h->xi32);
-    dfree(h->coef32);
+    dfree(h->bnd);
     h->Nb = 0;
     h->coef_dirty = true;
     h->have_scr = false;
@@ -81,6 +80,9 @@ void free_particles(pd_handle *h)
     dfree(h->dmg);
     dfree(h->minv32);
     dfree(h->theta32);
+    dfree(h->rec);
+    h->rec_u_valid = false;
+    dfree(h->order);
     dfree(h->row_ptr);
 }
 
@@ -100,23 +102,32 @@ int ensure_coef(pd_handle *h)
     if (!h->coef_dirty) return PD_OK;
     hipStream_t s = h->stream;
     const pd_params &P = h->P;
+    if (P.precision == PD_PREC_MIXED) {
+        LAUNCH(dalloc(&h->rec, (size_t)h->N * 8));
+        LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec));
+        // tiled processing order: 16 dx tiles in 3D (a tile plus its horizon
+        // halo is ~0.7 MB of records, inside one XCD's 4 MB L2), 64 dx in 2D
+        int tc = h->tile_cells < 0 ? (P.dim == 3 ? 16 : 64) : h->tile_cells;
+        dfree(h->order);
+        if (tc > 0) {
+            int rc = build_tile_order(h, tc * P.dx);
+            if (rc) return rc;
+        }
+    }
     if (P.model == PD_MODEL_PMB) {
         LAUNCH(dalloc(&h->kappa, h->Nb));
-        LAUNCH(dalloc(&h->xi32, h->Nb));
-        LAUNCH(dalloc(&h->coef32, h->Nb));
-        LAUNCH(launch_pmb_coef(s, h->Nb, h->fac, h->scr, P.vol, P.bc, h->idist, h->kappa, h->xi32, h->coef32));
+        LAUNCH(dalloc(&h->bnd, h->Nb));
+        LAUNCH(launch_pmb_coef(s, h->Nb, h->fac, h->scr, P.vol, P.bc, h->idist, h->kappa, h->bnd));
     } else {
         if (!h->have_weights) {
             h->err = "LPS force needs pd_lps_weights first";
             return PD_E_INVALID;
         }
         if (P.precision == PD_PREC_MIXED) {
-            LAUNCH(dalloc(&h->xi32, h->Nb));
-            LAUNCH(dalloc(&h->coef32, h->Nb));
+            LAUNCH(dalloc(&h->bnd, h->Nb));
             LAUNCH(dalloc(&h->minv32, h->N));
             LAUNCH(dalloc(&h->theta32, h->N));
-            LAUNCH(launch_lps_coef(s, h->Nb, h->w, h->fac, P.vol, h->idist, h->xi32, h->coef32, h->N, h->m,
-                                   h->minv32));
+            LAUNCH(launch_lps_coef(s, h->Nb, h->w, h->fac, P.vol, h->idist, h->bnd, h->N, h->m, h->minv32));
         } else {
             LAUNCH(dalloc(&h->e, h->Nb));
         }
@@ -127,8 +138,9 @@ int ensure_coef(pd_handle *h)
 
 // Launch shape of the mixed kernels: lanes per particle from the mean row
 // length (the autotuner in pd_gpu1_amd/autotune.py can override it).
-int pick_group(const pd_handle *h)
+int pick_group(const pd_handle *h, int pass = 0)
 {
+    if (pass == 1 && h->group2) return h->group2;
     if (h->group) return h->group;
     double avg = h->N ? (double)h->Nb / h->N : 0.0;
     if (avg > 96) return 64;
@@ -156,6 +168,20 @@ void lps_mixed_consts(const pd_params &P, float *ctheta, float *A, float *B)
     }
 }
 
+// Mixed mode: bring the u half of the particle records up to date when some
+// call changed u without refreshing them (the integrators and the contact
+// kernel refresh it themselves).  Runs before a graph is captured or
+// replayed, so captured sequences never contain the pack pass.
+hipError_t sync_rec_u(pd_handle *h)
+{
+    if (h->P.precision != PD_PREC_MIXED || !h->rec || h->rec_u_valid) return hipSuccess;
+    hipError_t e = launch_pack_u(h->stream, nullptr, h->P.dim, h->N, cd3(h->u), h->rec);
+    if (e == hipSuccess) h->rec_u_valid = true;
+    return e;
+}
+
+double *rec_or_null(const pd_handle *h) { return h->P.precision == PD_PREC_MIXED ? h->rec : nullptr; }
+
 // Force evaluation (PMB, or LPS theta pass + force pass) on the handle's
 // stream.  stop may be null.
 int force_launch(pd_handle *h, const int *stop)
@@ -171,8 +197,9 @@ int force_launch(pd_handle *h, const int *stop)
             LAUNCH(launch_pmb_fp64(s, stop, P.dim, h->N, h->row_ptr, h->col, h->kappa, h->idist, h->fail,
                                    cd3(h->x), cd3(h->u), h->f, P.s0, ymin));
         } else {
-            LAUNCH(launch_pmb_mixed(s, stop, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->xi32, h->coef32,
-                                    h->fail, cd3(h->x), cd3(h->u), h->f32, P.s0, P.near_window, ymin));
+            LAUNCH(sync_rec_u(h));
+            LAUNCH(launch_pmb_mixed(s, stop, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->bnd,
+                                    h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
         }
     } else {
         // theta for i < N (Force_State.cu:339), force for i < N_int (:359)
@@ -187,12 +214,11 @@ int force_launch(pd_handle *h, const int *stop)
             const int g = pick_group(h);
             float ctheta, A, B;
             lps_mixed_consts(P, &ctheta, &A, &B);
-            LAUNCH(launch_lps_theta_mixed(s, stop, P.dim, g, h->N, h->row_ptr, h->col, h->xi32, h->coef32, h->fail,
-                                          cd3(h->x), cd3(h->u), h->minv32, ctheta, P.theta_skip_broken,
-                                          h->theta32));
-            LAUNCH(launch_lps_force_mixed(s, stop, P.dim, g, h->N_int, h->row_ptr, h->col, h->xi32, h->coef32,
-                                          h->fail, cd3(h->x), cd3(h->u), h->minv32, h->theta32, A, B, P.s0,
-                                          P.near_window, ymin, h->f32));
+            LAUNCH(sync_rec_u(h));
+            LAUNCH(launch_lps_theta_mixed(s, stop, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd, h->fail,
+                                          h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
+            LAUNCH(launch_lps_force_mixed(s, stop, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr, h->col, h->bnd,
+                                          h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
         }
     }
     return PD_OK;
@@ -204,7 +230,7 @@ int verlet1_launch(pd_handle *h)
 {
     const pd_params &P = h->P;
     LAUNCH(launch_verlet1(h->stream, P.dim, h->N_int, P.n_top, P.n_bot, P.dt, P.strip_velocity, h->v, cd3(h->a),
-                          h->u, h->have_cyl ? h->S->cyl : nullptr));
+                          h->u, h->have_cyl ? h->S->cyl : nullptr, rec_or_null(h)));
     return PD_OK;
 }
 
@@ -220,7 +246,7 @@ int contact_launch(pd_handle *h)
 {
     const pd_params &P = h->P;
     LAUNCH(launch_contact(h->stream, h->red_blocks, h->red_threads, h->N_int, h->dmg, cd3(h->x), h->u, h->v,
-                          h->cyl, P.dt, P.dens, P.vol, h->partials, h->S));
+                          h->cyl, P.dt, P.dens, P.vol, h->partials, h->S, rec_or_null(h)));
     return PD_OK;
 }
 
@@ -242,7 +268,7 @@ int adr_iter_launch(pd_handle *h, double tol)
                              cd3(h->u), cd3(h->f), cf3(h->f32), use32(h), cd3(h->fo), cd3(h->vho), cd3(h->b),
                              h->partials, h->S, tol));
     LAUNCH(launch_adr_update(h->stream, P.dim, h->N_int, P.mass, P.dt, cd3(h->f), cf3(h->f32), use32(h),
-                             cd3(h->b), h->vh, h->v, h->vho, h->u, h->fo, h->S));
+                             cd3(h->b), h->vh, h->v, h->vho, h->u, h->fo, h->S, rec_or_null(h)));
     return PD_OK;
 }
 
@@ -510,6 +536,7 @@ int pd_set_params(pd_handle *h, const pd_params *p)
     bool coef = p->vol != h->P.vol || p->bc != h->P.bc || p->precision != h->P.precision;
     h->P = *p;
     if (coef) h->coef_dirty = true;
+    h->rec_u_valid = false;  // FP64 mode moves u without refreshing the records
     drop_graph(h);
     return PD_OK;
 }
@@ -517,6 +544,7 @@ int pd_set_params(pd_handle *h, const pd_params *p)
 int pd_set_positions(pd_handle *h, const double *x, const double *y, const double *z, int n_int, int n)
 {
     if (!h) return PD_E_INVALID;
+    h->rec_u_valid = false;
     REQUIRE(x && y && (h->P.dim == 2 || z), "null coordinate array");
     REQUIRE(n > 0 && n_int > 0 && n_int <= n, "need 0 < n_int <= n");
     REQUIRE(n_int + h->P.n_top + h->P.n_bot <= n, "strip counts exceed n - n_int");
@@ -647,6 +675,7 @@ int pd_init_state(pd_handle *h, int mode)
     // and Fold.  dmg = 0 (the reference leaves it unset for KW; see
     // Solve_Kalthoff_Winkler.cu:122,187).
     if (!h) return PD_E_INVALID;
+    h->rec_u_valid = false;
     REQUIRE(h->col, "pd_build_neighbors first");
     hipStream_t s = h->stream;
     LAUNCH(launch_fill_u32(s, h->fail, fail_words(h), 0xFFFFFFFFu));
@@ -719,8 +748,13 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
     const double ymin = P.dim == 3 ? P.break_y_min : -INFINITY;
     double acc[2] = {0.0, 0.0};
     for (int r = 0; r < reps && rc == PD_OK; ++r) {
+        if (P.precision == PD_PREC_MIXED) LAUNCH(launch_pack_u(h->stream, nullptr, P.dim, h->N, cd3(h->u), h->rec));
         LAUNCH(hipEventRecord(e[0], h->stream));
-        if (P.model == PD_MODEL_PMB) {
+        if (P.model == PD_MODEL_PMB && P.precision == PD_PREC_MIXED) {
+            LAUNCH(launch_pmb_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->bnd, h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
+            LAUNCH(hipEventRecord(e[1], h->stream));
+            LAUNCH(hipEventRecord(e[2], h->stream));
+        } else if (P.model == PD_MODEL_PMB) {
             rc = force_launch(h, nullptr);
             LAUNCH(hipEventRecord(e[1], h->stream));
             LAUNCH(hipEventRecord(e[2], h->stream));
@@ -737,13 +771,10 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
             const int g = pick_group(h);
             float ctheta, A, B;
             lps_mixed_consts(P, &ctheta, &A, &B);
-            LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, P.dim, g, h->N, h->row_ptr, h->col, h->xi32, h->coef32,
-                                          h->fail, cd3(h->x), cd3(h->u), h->minv32, ctheta, P.theta_skip_broken,
-                                          h->theta32));
+            LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd,
+                                          h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
             LAUNCH(hipEventRecord(e[1], h->stream));
-            LAUNCH(launch_lps_force_mixed(h->stream, nullptr, P.dim, g, h->N_int, h->row_ptr, h->col, h->xi32,
-                                          h->coef32, h->fail, cd3(h->x), cd3(h->u), h->minv32, h->theta32, A, B, P.s0,
-                                          P.near_window, ymin, h->f32));
+            LAUNCH(launch_lps_force_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr, h->col, h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
             LAUNCH(hipEventRecord(e[2], h->stream));
         }
         LAUNCH(hipEventSynchronize(e[2]));
@@ -766,6 +797,69 @@ int pd_set_group(pd_handle *h, int group)
     REQUIRE(group == 0 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
             "group must be 0, 4, 8, 16, 32 or 64");
     h->group = group;
+    h->group2 = group;
+    drop_graph(h);
+    return PD_OK;
+}
+
+int pd_stream_copy(int device, int64_t bytes, int reps, double *gbps)
+{
+    // Device-to-device copy of `bytes` with a plain 16-byte-per-lane kernel:
+    // the reference point for HBM counters and achievable bandwidth.
+    if (bytes <= 0 || (bytes % 16) != 0 || reps <= 0 || !gbps) return PD_E_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return PD_E_HIP;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return PD_E_HIP;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return PD_E_HIP;
+    }
+    hipEvent_t e0, e1;
+    int rc = PD_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PD_E_HIP;
+    if (!rc && hipMemset(a, 1, bytes) != hipSuccess) rc = PD_E_HIP;
+    if (!rc && launch_copy16(0, a, b, bytes) != hipSuccess) rc = PD_E_HIP;  // warm
+    if (!rc) {
+        (void)hipEventRecord(e0, 0);
+        for (int r = 0; r < reps && !rc; r++)
+            if (launch_copy16(0, a, b, bytes) != hipSuccess) rc = PD_E_HIP;
+        (void)hipEventRecord(e1, 0);
+        float ms = 0.f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = PD_E_HIP;
+        if (!rc) *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
+int pd_set_pass_group(pd_handle *h, int pass, int group)
+{
+    // Lanes per particle of one pass of the mixed kernels (pass 0: PMB or
+    // LPS dilatation, pass 1: LPS force); pd_set_group sets both.
+    if (!h) return PD_E_INVALID;
+    REQUIRE(pass == 0 || pass == 1, "pass must be 0 or 1");
+    REQUIRE(group == 0 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
+            "group must be 0, 4, 8, 16, 32 or 64");
+    if (pass == 0)
+        h->group = group;
+    else
+        h->group2 = group;
+    drop_graph(h);
+    return PD_OK;
+}
+
+int pd_set_tile(pd_handle *h, int tile_cells)
+{
+    // Edge (in units of dx) of the tiles that order the throughput kernels'
+    // work; 0 processes particles in index order, -1 restores the default.
+    if (!h) return PD_E_INVALID;
+    REQUIRE(tile_cells >= -1 && tile_cells <= 4096, "tile_cells out of range");
+    h->tile_cells = tile_cells;
+    h->coef_dirty = true;
     drop_graph(h);
     return PD_OK;
 }
@@ -809,6 +903,7 @@ int pd_run_verlet(pd_handle *h, int nsteps)
     if (nsteps == 0) return PD_OK;
     int rc = ensure_coef(h);
     if (rc) return rc;
+    LAUNCH(sync_rec_u(h));
     const int kind = 1 + (h->have_cyl ? 1 : 0);
     rc = capture(h, kind, 1, [&]() { return step_launch(h); });
     if (rc) return rc;
@@ -861,6 +956,7 @@ int pd_adr_solve(pd_handle *h, int ct0, int max_iters, double tol, int *iters_ou
     REQUIRE(h->col && max_iters >= 0 && tol >= 0.0, "bonds, max_iters >= 0 and tol >= 0 required");
     int rc = ensure_coef(h);
     if (rc) return rc;
+    LAUNCH(sync_rec_u(h));
     LAUNCH(launch_set_ct(h->stream, h->S, ct0, 1));
     const int chunk = 32;
     int launched = 0;
@@ -899,6 +995,7 @@ int pd_strip_displace(pd_handle *h, double du_top, double du_bot)
 {
     if (!h) return PD_E_INVALID;
     REQUIRE(h->N > 0, "pd_set_positions first");
+    h->rec_u_valid = false;
     LAUNCH(launch_strip_displace(h->stream, h->N_int, h->P.n_top, h->P.n_bot, du_top, du_bot, h->u.p[1]));
     return PD_OK;
 }
@@ -995,6 +1092,7 @@ int pd_set_field(pd_handle *h, int field, const double *in)
     REQUIRE(field_ptr(h, field, &p, &n, &vec) == PD_OK, "unknown field");
     REQUIRE(p != nullptr, "field not allocated");
     REQUIRE(field != PD_FIELD_X0, "reference coordinates are set by pd_set_positions");
+    if (field == PD_FIELD_U) h->rec_u_valid = false;
     LAUNCH(hipStreamSynchronize(h->stream));
     if (vec) {
         for (int d = 0; d < h->P.dim; d++)
@@ -1148,6 +1246,7 @@ int pd_checkpoint_read(pd_handle *h, const char *path, int64_t *step)
 {
     if (!h || !path) return PD_E_INVALID;
     REQUIRE(h->fail, "pd_build_neighbors first");
+    h->rec_u_valid = false;
     const int D = h->P.dim, N = h->N;
     FILE *fp = std::fopen(path, "rb");
     REQUIRE(fp, "cannot open checkpoint file");

@@ -46,6 +46,10 @@ struct pd_handle {
     F3 f32{};                 // mixed-mode force
     double *m = nullptr, *theta = nullptr, *dmg = nullptr;
     float *minv32 = nullptr, *theta32 = nullptr;  // mixed LPS: 1/m and dilatation
+    double *rec = nullptr;        // mixed mode: 64-byte particle records (x0 | theta/m, 1/m | u)
+    bool rec_u_valid = false;     // the u half of every record equals u
+    int32_t *order = nullptr;     // mixed mode: tiled processing order (null = index order)
+    int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
     double *react[3] = {nullptr, nullptr, nullptr};
 
     // bonds (CSR)
@@ -54,8 +58,8 @@ struct pd_handle {
     uint32_t *fail = nullptr;
     double *idist = nullptr, *fac = nullptr, *scr = nullptr, *kappa = nullptr;
     double *w = nullptr, *e = nullptr;
-    float *xi32 = nullptr, *coef32 = nullptr;
-    bool coef_dirty = true;   // kappa / coef32 need recomputation
+    float2 *bnd = nullptr;    // mixed mode: per bond {|xi|, kappa or omega} in FP32
+    bool coef_dirty = true;   // kappa / bnd need recomputation
     bool have_scr = false;
     bool have_weights = false;
 
@@ -73,7 +77,8 @@ struct pd_handle {
     bool have_cyl = false;
 
     // force launch shape (autotuned group size)
-    int group = 0;
+    int group = 0;   // pass 0 (PMB or LPS dilatation); 0 = built-in choice
+    int group2 = 0;  // pass 1 (LPS force)
 
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -99,6 +104,7 @@ struct pd_handle {
 
 // k_neighbor.hip
 int build_neighbors_impl(pd_handle *h);
+int build_tile_order(pd_handle *h, double tile_len);
 hipError_t launch_cell_ids(hipStream_t s, int dim, int N, CD3 x, const double *lo, double cs,
                            const int *ncell, uint32_t *cell_of, uint32_t *idx);
 hipError_t launch_cell_bounds(hipStream_t s, int N, const uint32_t *sorted_cell, int ncells,
@@ -129,10 +135,10 @@ hipError_t launch_crack_kw(hipStream_t s, int N_int, const int32_t *rp, const in
 hipError_t launch_lps_weights(hipStream_t s, int N, const int32_t *rp, const double *idist,
                               const double *fac, double delta, double vol, double *w, double *m);
 hipError_t launch_pmb_coef(hipStream_t s, int64_t nb, const double *fac, const double *scr, double vol,
-                           double bc, const double *idist, double *kappa, float *xi32, float *coef32);
+                           double bc, const double *idist, double *kappa, float2 *bnd);
 hipError_t launch_lps_coef(hipStream_t s, int64_t nb, const double *w, const double *fac, double vol,
-                           const double *idist, float *xi32, float *coef32, int N, const double *m,
-                           float *minv32);
+                           const double *idist, float2 *bnd, int N, const double *m, float *minv32);
+hipError_t launch_copy16(hipStream_t s, const void *a, void *b, int64_t bytes);
 hipError_t launch_damage(hipStream_t s, int N_int, const int32_t *rp, const uint32_t *fail,
                          const double *fac, double vol, double *dmg);
 
@@ -140,9 +146,6 @@ hipError_t launch_damage(hipStream_t s, int N_int, const int32_t *rp, const uint
 hipError_t launch_pmb_fp64(hipStream_t s, const int *stop, int dim, int n_rows, const int32_t *rp, const int32_t *col,
                            const double *kappa, const double *idist, uint32_t *fail, CD3 x, CD3 u,
                            D3 f, double s0, double ymin);
-hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, int dim, int group, int n_rows, const int32_t *rp,
-                            const int32_t *col, const float *xi32, const float *kap32, uint32_t *fail,
-                            CD3 x, CD3 u, F3 f, double s0, double window, double ymin);
 hipError_t launch_lps_theta_fp64(hipStream_t s, const int *stop, int dim, int N, const int32_t *rp, const int32_t *col,
                                  CD3 x, CD3 u, const double *idist, const double *fac, const double *w,
                                  const double *m, const uint32_t *fail, double vol, double pratio,
@@ -152,18 +155,23 @@ hipError_t launch_lps_force_fp64(hipStream_t s, const int *stop, int dim, int N_
                                  const double *m, const double *theta, const double *e, double vol,
                                  double pratio, double K, double G, double s0, double ymin,
                                  uint32_t *fail, D3 f);
-hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, int dim, int group, int N, const int32_t *rp,
-                                  const int32_t *col, const float *xi32, const float *om32,
-                                  const uint32_t *fail, CD3 x, CD3 u, const float *m32, float ctheta,
-                                  int skip_broken, float *theta32);
-hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, int dim, int group, int N_int, const int32_t *rp,
-                                  const int32_t *col, const float *xi32, const float *om32, uint32_t *fail,
-                                  CD3 x, CD3 u, const float *m32, const float *theta32, float A, float B,
-                                  double s0, double window, double ymin, F3 f);
+
+// k_force_mixed.hip (particle records: 8 doubles per particle, x0 | u | (theta/m, 1/m))
+hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double *rec);
+hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, double *rec);
+hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
+                            const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
+                            const double *rec, F3 f, double s0, double window, double ymin);
+hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
+                                  const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
+                                  double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32);
+hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
+                                  const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
+                                  const double *rec, float A, float B, double s0, double window, double ymin, F3 f);
 
 // k_integrate.hip
 hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
-                          D3 v, CD3 a, D3 u, double *cyl_state);
+                          D3 v, CD3 a, D3 u, double *cyl_state, double *rec);
 hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f,
                           CF3 f32, int use32, CD3 b);
 hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, int N_int, int N,
@@ -171,10 +179,10 @@ hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, in
                              CD3 b, double *partials, DevScalars *S, double tol);
 hipError_t launch_adr_update(hipStream_t s, int dim, int N_int, double mass, double dt, CD3 f,
                              CF3 f32, int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo,
-                             DevScalars *S);
+                             DevScalars *S, double *rec);
 hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u,
                           D3 v, pd_cyl cyl, double dt, double dens, double vol, double *partials,
-                          DevScalars *S);
+                          DevScalars *S, double *rec);
 hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done);
 hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double *v, int64_t n,
                             double *partials, DevScalars *S);

@@ -34,6 +34,13 @@ def params(dim, model, dt, dx, thick=0.0, length=0.05, height=0.05, width=0.0, e
     return p, (nxyz[0], nxyz[1], nxyz[2]), n.value
 
 
+def stream_copy(nbytes, reps=10, device=0):
+    """Achieved GB/s of a plain device copy (HBM reference)."""
+    out = C.c_double()
+    _check(None, lib().pd_stream_copy(device, nbytes, reps, C.byref(out)))
+    return out.value
+
+
 def gen_plate(nx, ny, dx):
     x, y = np.zeros(nx * ny), np.zeros(nx * ny)
     _check(None, lib().pd_gen_plate(nx, ny, dx, dptr(x), dptr(y)))
@@ -160,6 +167,12 @@ class Problem:
 
     def set_group(self, group):
         self._c(lib().pd_set_group(self._h, group))
+
+    def set_pass_group(self, pass_, group):
+        self._c(lib().pd_set_pass_group(self._h, pass_, group))
+
+    def set_tile(self, tile_cells):
+        self._c(lib().pd_set_tile(self._h, tile_cells))
 
     def verlet_half1(self):
         self._c(lib().pd_verlet_half1(self._h))

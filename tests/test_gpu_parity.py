@@ -281,3 +281,72 @@ def test_mixed_lps3d_force():
         assert np.max(np.abs(f_g - f_o)) / np.max(np.abs(f_o)) < 5e-5
         assert np.array_equal(prob.fail_mask(), m.fail)
         assert (m.fail == 0).sum() > 0
+
+
+@pytest.mark.parametrize("model", ["pmb", "lps"])
+def test_mixed_group_sizes_agree(model):
+    if model == "pmb":
+        sc = scenarios.pmb_block(nx=24, ny=22, nz=18, precision=_lib.PD_PREC_MIXED)
+    else:
+        sc = scenarios.c2_lps_block(nx=24, ny=22, nz=18, precision=_lib.PD_PREC_MIXED)
+    rng = np.random.default_rng(5)
+    u = np.stack([2.03e-3 * sc.x, -3e-4 * sc.y, 1e-3 * sc.z]) + rng.uniform(-2e-9, 2e-9, (3, len(sc.x)))
+    ref_f, ref_mask = None, None
+    for g in (4, 8, 16, 32, 64):
+        with sc.make() as prob:
+            prob.set_group(g)
+            prob.set("U", u)
+            prob.force()
+            f, mask = prob.get("F"), prob.fail_mask()
+        if ref_f is None:
+            ref_f, ref_mask = f, mask
+        else:
+            assert np.max(np.abs(f - ref_f)) <= 1e-6 * np.max(np.abs(ref_f)), g
+            assert np.array_equal(mask, ref_mask), g
+    assert (ref_mask == 0).sum() > 0
+
+
+def test_mixed_tile_order_does_not_change_results():
+    # strips after N_int: the tiled order must keep integrated rows first
+    sc = scenarios.c0_plate_pmb(scale=0.6)
+    rng = np.random.default_rng(9)
+    u = rng.uniform(-3e-6, 3e-6, size=(2, len(sc.x)))
+    outs = []
+    for tile in (0, 8, 64):
+        with sc.make(precision=_lib.PD_PREC_MIXED) as prob:
+            prob.set_tile(tile)
+            prob.set("U", u)
+            prob.force()
+            outs.append((prob.get("F"), prob.fail_mask()))
+    for f, mk in outs[1:]:
+        assert np.array_equal(f, outs[0][0]) and np.array_equal(mk, outs[0][1])
+
+
+def test_mixed_records_follow_every_displacement_update():
+    # The integrators refresh the u half of the particle records in place;
+    # set("U"), strip_displace and a precision switch must force a re-pack.
+    sc = scenarios.c0_plate_pmb(scale=0.6)
+    rng = np.random.default_rng(21)
+    u0 = rng.uniform(-2e-6, 2e-6, size=(2, len(sc.x)))
+    u1 = rng.uniform(-2e-6, 2e-6, size=(2, len(sc.x)))
+    with sc.make(precision=_lib.PD_PREC_MIXED) as a, sc.make(precision=_lib.PD_PREC_MIXED) as b:
+        for u in (u0, u1):
+            a.set("U", u)
+            b.set("U", u)
+            a.run_verlet(15)  # captured graph
+            for _ in range(15):  # staged calls
+                b.verlet_half1()
+                b.force()
+                b.verlet_half2()
+            assert np.array_equal(a.get("U"), b.get("U"))
+            assert np.array_equal(a.get("F"), b.get("F"))
+            assert np.array_equal(a.fail_mask(), b.fail_mask())
+        a.strip_displace(1e-6, -1e-6)
+        uf, mask = a.get("U"), a.fail_mask()  # bonds that break in this pass still act in it
+        a.force()
+        f_a = a.get("F")
+    with sc.make(precision=_lib.PD_PREC_MIXED) as c:
+        c.set("U", uf)
+        c.set_fail_mask(mask)
+        c.force()
+        assert np.array_equal(c.get("F"), f_a)
