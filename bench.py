@@ -14,6 +14,13 @@ the timed region starts.  The roofline object reports the dominant kernel
 (the LPS force pass): algorithmic bytes per launch / its mean HIP-event
 duration against the 8 TB/s HBM3E peak.  cpu_baseline times the FP64 oracle on
 the host cores over a bounded sub-block of the same workload.
+
+The "adr" object measures the second half of the metric, ADR wall time to a
+1e-6 relative residual, on 2D PMB plates loaded in 1e-6 m strip increments:
+the batched persistent ADR over a seeded plate family (sides 40..200
+particles), and plain versus warm-started ADR on unseen plates (sides
+200..400) over consecutive increments, with and without the trained
+network.
 """
 import argparse
 import json
@@ -87,6 +94,48 @@ def cpu_baseline(sample_dims=(200, 200, 100), evals=8):
     }
 
 
+def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6):
+    """Batched ADR timing and the warm-start comparison (see module doc)."""
+    import numpy as np
+    from pd_gpu1_amd import batch as PB
+    from pd_gpu1_amd import warmstart as W
+    from pd_gpu1_amd.mpnn import Mpnn
+
+    out = {}
+    specs = PB.plate_family(n_batch)
+    with PB.Batch(PB.plate_params(), specs) as b:
+        n_part, n_bonds, _ = b.info()
+        incs = []
+        for _ in range(batch_increments):
+            its, res, ms = b.increment(W.DU, 1e-6, 400000, block=1024)
+            incs.append({"wall_ms": round(ms, 1), "iterations_median": int(np.median(its)),
+                         "iterations_max": int(its.max()), "max_residual": float(res.max())})
+    out["batched"] = {"workload": f"{n_batch} PMB plates, sides 40..200 (seed 0x5EED0003+k), du=1e-6 m, "
+                                  "ADR to relative residual 1e-6, one persistent launch per increment",
+                      "particles": n_part, "bonds": n_bonds, "increments": incs}
+    wpath = os.path.join(ROOT, "pd_gpu1_amd", "data", "mpnn_warmstart.npy")
+    specs = W.eval_specs(n_eval)
+    with Mpnn() as net:
+        have_net = os.path.exists(wpath)
+        if have_net:
+            net.set_params(np.load(wpath))
+        rows = np.array(W.evaluate(net, specs, eval_increments), dtype=float)
+    tot = lambda sel, c: float(rows[sel, c].sum())
+    every, first, later = rows[:, 1] >= 0, rows[:, 1] == 0, rows[:, 1] >= 1
+    out["warm_start"] = {
+        "workload": f"{n_eval} unseen PMB plates (seed 0x5EED1004+k, sides {[int(s[0]) for s in specs]}), "
+                    f"{eval_increments} consecutive 1e-6 m increments each, ADR to relative residual 1e-6",
+        "plain_s": tot(every, 5), "warm_no_network_s": tot(every, 6), "warm_network_s": tot(every, 7),
+        "speedup_no_network": tot(every, 5) / tot(every, 6), "speedup_network": tot(every, 5) / tot(every, 7),
+        "iterations": {"plain": tot(every, 2), "warm_no_network": tot(every, 3), "warm_network": tot(every, 4)},
+        "first_increment_iterations": {"plain": tot(first, 2), "uniform_stretch": tot(first, 3),
+                                       "network": tot(first, 4), "network_fell_back": int(rows[first, 9].sum())},
+        "later_increment_iterations": {"plain": tot(later, 2), "warm": tot(later, 3)},
+        "network_weights": "pd_gpu1_amd/data/mpnn_warmstart.npy" if have_net else None,
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +143,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--reps", type=int, default=20, help="launches per pass for the kernel timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adr", action="store_true", help="skip the ADR / warm-start measurements")
     ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter reference)")
@@ -163,7 +213,7 @@ def main():
                        "bytes_per_launch": b_theta, "ms_per_launch": ms_theta},
     }
     out = {
-        "metric": "bond-updates/s & %HBM peak on 1 MI355X",
+        "metric": "bond-updates/s & %HBM peak on 1 MI355X; ADR wall time to 1e-6 resid vs plain",
         "value": value,
         "unit": "bond-updates/s",
         "n_gpus": 1,
@@ -184,6 +234,10 @@ def main():
                    "broken_bonds_after_run": broken},
         "roofline": roofline,
     }
+    if not args.no_adr:
+        t = time.perf_counter()
+        out["adr"] = adr_section()
+        print(f"[bench] ADR measurements {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -10,9 +10,10 @@
 // record.
 //
 // Work split.  A group of G lanes (G in 4..64, a divisor of the wave64) owns
-// one particle; lanes stride over its CSR row two bonds at a time so that
-// col, xi, the bond coefficient and the fail word of both bonds, and both
-// neighbour records, are in flight together.  Blocks are renumbered so that
+// one particle; lanes stride over its CSR row two bonds at a time.  The bond
+// stream (col, {|xi|, coefficient}, fail word) of the next pair is fetched
+// while the neighbour records of the current pair are gathered, and broken
+// or absent bonds are masked arithmetically instead of branched around.  Blocks are renumbered so that
 // each of the 8 XCDs (which receive workgroups round-robin) walks one
 // contiguous eighth of the particle range, keeping neighbour records in that
 // XCD's L2.
@@ -20,12 +21,12 @@
 // Arithmetic.  The stretch is formed without cancellation from
 // eta = u_j - u_i and xi = x0_j - x0_i (both differenced in FP64, then FP32):
 //     s = eta.(2 xi + eta) / (|xi| (|xi + eta| + |xi|)),
-// equal to (|xi+eta| - |xi|)/|xi| of Force_Bond.cu:189.  Per-lane sums use
-// Neumaier compensation; the group sum combines lanes with error-free TwoSum
-// steps over xor shuffles.  A bond whose FP32 stretch lies within
-// near_window*s0 of s0 is re-evaluated with the FP64 parity expression
-// (Force_Bond.cu:185-189, Surf_Corr.cu:324-326) before its break is decided,
-// so breaks follow the FP64 stretch of the current state.
+// equal to (|xi+eta| - |xi|)/|xi| of Force_Bond.cu:189.  Bond terms are FP32;
+// per-lane sums accumulate them in FP64 (exact for each term) and the group
+// sum reduces the FP64 partials over xor shuffles.  A bond whose FP32 stretch
+// lies within near_window*s0 of s0 is re-evaluated with the FP64 parity
+// expression (Force_Bond.cu:185-189, Surf_Corr.cu:324-326) before its break
+// is decided, so breaks follow the FP64 stretch of the current state.
 #include "pd_internal.h"
 
 namespace {
@@ -79,7 +80,9 @@ __device__ __forceinline__ float group_sum(double a)
     return (float)a;
 }
 
-// Geometry of one bond in FP32 from FP64 differences.
+// Geometry of one bond in FP32 from FP64 differences.  Only this FP32 part
+// may contract a*b+c into FMA; the FP64 parity stretch below keeps the
+// separate roundings of the oracle.
 template <int DIM>
 struct BondGeo {
     float dx, dy, dz;   // xi + eta
@@ -91,6 +94,7 @@ struct BondGeo {
 template <int DIM>
 __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
 {
+#pragma clang fp contract(fast)
     const float ex = (float)(rc.x0 - ri.x0), ey = (float)(rc.x1 - ri.x1);
     const float hx = (float)(rc.u0 - ri.u0), hy = (float)(rc.u1 - ri.u1);
     BondGeo<DIM> g;
@@ -129,23 +133,65 @@ __device__ __forceinline__ double stretch64(const Rec &ri, const Rec &rc)
     return (sqrt(r2) - id) / id;
 }
 
-template <int DIM>
-__device__ __forceinline__ void pmb_bond(const Rec &ri, const Rec &rc, float L0, float kap, bool ok, int32_t k,
-                                         float s0f, float win, double s0, double ymin, double &ax, double &ay,
-                                         double &az, uint32_t *fail)
+// One bond slot of a lane: neighbour index, |xi| and bond coefficient, and a
+// mask that is all ones when the slot holds a bond that acts in this pass.
+// Slots past the end of the row point at the particle itself with a zero
+// mask, so every lane runs the same branch-free arithmetic.
+struct Slot {
+    int c;
+    float L0, coef;
+    uint32_t mask;
+};
+
+__device__ __forceinline__ Slot fetch_slot(int32_t k, int32_t ke, int self, const int32_t *__restrict__ col,
+                                           const float2 *__restrict__ bnd, const uint32_t *__restrict__ fail,
+                                           bool need_intact)
 {
-    const BondGeo<DIM> g = geo<DIM>(ri, rc);
-    const float s = g.num * frcp(L0 * (g.L + L0));
-    const float t = ok ? kap * s * g.rL : 0.f;
-    ax += (double)(t * g.dx);
-    ay += (double)(t * g.dy);
-    if (DIM == 3) az += (double)(t * g.dz);
-    if (ok && s >= s0f - win) {
-        const bool brk = (s > s0f + win ? true : stretch64<DIM, false>(ri, rc) > s0) &&
+    const bool valid = k < ke;
+    const int32_t kk = valid ? k : 0;
+    const int c = col[kk];
+    const float2 b = bnd[kk];
+    const uint32_t w = fail[kk >> 5];
+    const bool act = valid && (!need_intact || ((w >> (k & 31)) & 1u));
+    return Slot{valid ? c : self, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
+}
+
+__device__ __forceinline__ float masked(float v, uint32_t mask)
+{
+    return __uint_as_float(__float_as_uint(v) & mask);
+}
+
+template <int DIM, bool LPS3Z>
+__device__ __forceinline__ void check_break(const Rec &ri, const Rec &rc, float s, const Slot &sl, int32_t k,
+                                            float s0f, float win, double s0, double ymin, uint32_t *fail)
+{
+    if (sl.mask && s >= s0f - win) {
+        const bool brk = (s > s0f + win ? true : stretch64<DIM, LPS3Z>(ri, rc) > s0) &&
                          (DIM == 2 || (ri.x1 >= ymin && rc.x1 >= ymin));
         if (brk) bond_break(fail, k);
     }
 }
+
+// Row walk shared by the three kernels: lanes take bonds k, k+G of the row,
+// then k+2G, k+3G, ...; the slots of the next pair are fetched while the
+// neighbour records of the current pair are in flight.
+#define PD_ROW_WALK(NEED_INTACT, BODY)                                                           \
+    {                                                                                            \
+        const int32_t kb = rp[i], ke = rp[i + 1];                                                \
+        int32_t k = kb + lane;                                                                   \
+        Slot sa = fetch_slot(k, ke, i, col, bnd, fail, NEED_INTACT);                             \
+        Slot sb = fetch_slot(k + G, ke, i, col, bnd, fail, NEED_INTACT);                         \
+        while (k < ke) {                                                                         \
+            const RecQ ra = load_rec(rec, sa.c), rb = load_rec(rec, sb.c);                       \
+            const Slot na = fetch_slot(k + 2 * G, ke, i, col, bnd, fail, NEED_INTACT);           \
+            const Slot nb = fetch_slot(k + 3 * G, ke, i, col, bnd, fail, NEED_INTACT);           \
+            BODY(ra, sa, k);                                                                     \
+            BODY(rb, sb, k + G);                                                                 \
+            sa = na;                                                                             \
+            sb = nb;                                                                             \
+            k += 2 * G;                                                                          \
+        }                                                                                        \
+    }
 
 template <int DIM, int G>
 __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop, const int32_t *__restrict__ order, int n_rows,
@@ -160,17 +206,19 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
     const int i = order ? order[gid] : gid;
     const Rec ri = load_rec(rec, i).r;
     double ax = 0.0, ay = 0.0, az = 0.0;
-    const int32_t kb = rp[i], ke = rp[i + 1];
-    for (int32_t k = kb + lane; k < ke; k += 2 * G) {
-        const bool two = k + G < ke;
-        const int32_t k2 = two ? k + G : k;
-        const int c1 = col[k], c2 = col[k2];
-        const float2 b1 = bnd[k], b2 = bnd[k2];
-        const uint32_t w1 = fail[k >> 5], w2 = fail[k2 >> 5];
-        const Rec r1 = load_rec(rec, c1).r, r2 = load_rec(rec, c2).r;
-        pmb_bond<DIM>(ri, r1, b1.x, b1.y, bit_of(w1, k), k, s0f, win, s0, ymin, ax, ay, az, fail);
-        if (two) pmb_bond<DIM>(ri, r2, b2.x, b2.y, bit_of(w2, k2), k2, s0f, win, s0, ymin, ax, ay, az, fail);
+#define PMB_BODY(R, S, K)                                                                        \
+    {                                                                                            \
+        _Pragma("clang fp contract(fast)")                                                       \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const float s = g.num * frcp(S.L0 * (g.L + S.L0));                                       \
+        const float t = masked(S.coef * s * g.rL, S.mask);                                       \
+        ax += (double)(t * g.dx);                                                                \
+        ay += (double)(t * g.dy);                                                                \
+        if (DIM == 3) az += (double)(t * g.dz);                                                  \
+        check_break<DIM, false>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail);                     \
     }
+    PD_ROW_WALK(true, PMB_BODY)
+#undef PMB_BODY
     const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
     const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
     if (lane == 0) {
@@ -178,14 +226,6 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
         f.p[1][i] = sy;
         if (DIM == 3) f.p[2][i] = sz;
     }
-}
-
-template <int DIM>
-__device__ __forceinline__ void theta_bond(const Rec &ri, const Rec &rc, float L0, float om, bool ok, double &acc)
-{
-    const BondGeo<DIM> g = geo<DIM>(ri, rc);
-    const float e = g.num * frcp(g.L + L0);  // |xi+eta| - |xi|
-    acc += (double)(ok ? om * L0 * e : 0.f);
 }
 
 template <int DIM, int G>
@@ -204,43 +244,22 @@ __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__
     const int i = order ? order[gid] : gid;
     const Rec ri = load_rec(rec, i).r;
     double acc = 0.0;
-    const int32_t kb = rp[i], ke = rp[i + 1];
-    for (int32_t k = kb + lane; k < ke; k += 2 * G) {
-        const bool two = k + G < ke;
-        const int32_t k2 = two ? k + G : k;
-        const int c1 = col[k], c2 = col[k2];
-        const float2 b1 = bnd[k], b2 = bnd[k2];
-        const uint32_t w1 = fail[k >> 5], w2 = fail[k2 >> 5];
-        const Rec r1 = load_rec(rec, c1).r, r2 = load_rec(rec, c2).r;
-        theta_bond<DIM>(ri, r1, b1.x, b1.y, !skip_broken || bit_of(w1, k), acc);
-        if (two) theta_bond<DIM>(ri, r2, b2.x, b2.y, !skip_broken || bit_of(w2, k2), acc);
+    const bool need_intact = skip_broken != 0;
+#define THETA_BODY(R, S, K)                                                                      \
+    {                                                                                            \
+        _Pragma("clang fp contract(fast)")                                                       \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const float e = g.num * frcp(g.L + S.L0); /* |xi+eta| - |xi| */                          \
+        acc += (double)masked(S.coef * S.L0 * e, S.mask);                                        \
     }
+    PD_ROW_WALK(need_intact, THETA_BODY)
+#undef THETA_BODY
     const float th = group_sum<G>(acc);
     if (lane == 0) {
         const float r = minv32[i];
         const float theta = ctheta * r * th;
         theta32[i] = theta;
         *reinterpret_cast<float2 *>(rec + (size_t)i * kRec + 3) = make_float2(theta * r, r);
-    }
-}
-
-template <int DIM>
-__device__ __forceinline__ void lps_bond(const Rec &ri, const RecQ &rc, float qi, float ri_m, float L0, float om,
-                                         bool ok, int32_t k, float A, float B, float s0f, float win, double s0,
-                                         double ymin, double &ax, double &ay, double &az, uint32_t *fail)
-{
-    const BondGeo<DIM> g = geo<DIM>(ri, rc.r);
-    const float e = g.num * frcp(g.L + L0);
-    const float s = e * frcp(L0);
-    // (t_i + t_j) vol fac = omega [A xi (q_i + q_j) + B e (r_i + r_j)]
-    const float tt = ok ? om * (A * L0 * (qi + rc.q) + B * e * (ri_m + rc.m)) * g.rL : 0.f;
-    ax += (double)(tt * g.dx);
-    ay += (double)(tt * g.dy);
-    if (DIM == 3) az += (double)(tt * g.dz);
-    if (ok && s >= s0f - win) {
-        const bool brk = (s > s0f + win ? true : stretch64<DIM, true>(ri, rc.r) > s0) &&
-                         (DIM == 2 || (ri.x1 >= ymin && rc.r.x1 >= ymin));
-        if (brk) bond_break(fail, k);
     }
 }
 
@@ -258,20 +277,23 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
     if (gid >= N_int) return;
     const int i = order ? order[gid] : gid;
     const RecQ qi = load_rec(rec, i);
+    const Rec &ri = qi.r;
     double ax = 0.0, ay = 0.0, az = 0.0;
-    const int32_t kb = rp[i], ke = rp[i + 1];
-    for (int32_t k = kb + lane; k < ke; k += 2 * G) {
-        const bool two = k + G < ke;
-        const int32_t k2 = two ? k + G : k;
-        const int c1 = col[k], c2 = col[k2];
-        const float2 b1 = bnd[k], b2 = bnd[k2];
-        const uint32_t w1 = fail[k >> 5], w2 = fail[k2 >> 5];
-        const RecQ r1 = load_rec(rec, c1), r2 = load_rec(rec, c2);
-        lps_bond<DIM>(qi.r, r1, qi.q, qi.m, b1.x, b1.y, bit_of(w1, k), k, A, B, s0f, win, s0, ymin, ax, ay, az, fail);
-        if (two)
-            lps_bond<DIM>(qi.r, r2, qi.q, qi.m, b2.x, b2.y, bit_of(w2, k2), k2, A, B, s0f, win, s0, ymin, ax, ay, az,
-                          fail);
+    // (t_i + t_j) vol fac = omega [A xi (q_i + q_j) + B e (r_i + r_j)]
+#define LPSF_BODY(R, S, K)                                                                       \
+    {                                                                                            \
+        _Pragma("clang fp contract(fast)")                                                       \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const float e = g.num * frcp(g.L + S.L0);                                                \
+        const float s = e * frcp(S.L0);                                                          \
+        const float tt = masked(S.coef * (A * S.L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        ax += (double)(tt * g.dx);                                                               \
+        ay += (double)(tt * g.dy);                                                               \
+        if (DIM == 3) az += (double)(tt * g.dz);                                                 \
+        check_break<DIM, true>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail);                      \
     }
+    PD_ROW_WALK(true, LPSF_BODY)
+#undef LPSF_BODY
     const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
     const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
     if (lane == 0) {

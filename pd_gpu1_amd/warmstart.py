@@ -1,0 +1,305 @@
+"""Learned warm start for quasi-static ADR load stepping (no reference
+counterpart; the reference restarts ADR from the previous solution,
+Solve_Plate_Tensile.cu:108-131).
+
+A plate with a pre-crack is loaded in displacement increments: the top strip
+rows move by +du and the bottom rows by -du, then ADR relaxes the free
+particles until the relative residual is below tol.  Plain ADR starts each
+increment from the previous solution with only the strips moved.  The warm
+start also moves the free particles by a predicted increment:
+
+* while no bond broke in the previous increment, the response is linear in
+  the load and the previous increment du_prev is the prediction;
+* for the first increment of a plate (no history) and after an increment in
+  which bonds broke, the prediction is the uniform-stretch field between the
+  strips, du_base = (0, du * y / y_strip), plus the correction the
+  message-passing network (pd_mpnn_*) predicts from the plate's bond graph:
+  du_pred = du_base + du * (0, y_net_y).  Without the network (the baseline
+  the network is judged against) du_base alone is used.  Only the y part of
+  the prediction is applied: measured on unseen plates, any error in the
+  x displacement of the guess (even with rigid-body modes removed) leaves a
+  slowly damped mode that multiplies the ADR iteration count by 4-20, while
+  the y part alone is harmless.
+
+ADR then converges from that guess to the same tolerance, so accuracy is
+unchanged and only the iteration count moves.  Fallback: when the warm run's
+residual after ``probe`` iterations exceeds the residual plain ADR had after
+the same number of iterations, the increment restarts from the plain initial
+state.
+
+Node features [N][8]: position relative to the plate centre / half side (2),
+du_prev / du (2), damage, top-strip flag, bottom-strip flag, 1.  Edge
+features [E][3]: reference bond vector / delta (2), intact flag.  Targets
+[N][2]: (du_next - du_base) / du, weighted 1 on free particles and 0 on the
+strips (their motion is prescribed).  Training samples are the increments the network is
+used for: first increments and increments that follow bond breakage.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from . import batch as PB
+from .engine import Problem, gen_plate_strips
+from .mpnn import Mpnn
+
+DU = 1.0e-6
+DX = 0.5e-3
+
+
+def plate_layout(n):
+    """Particles of an n x n plate with 3 strip rows above and below (Coord.cu:47-66):
+    free particles first, then the top strip rows, then the bottom ones."""
+    x, y = gen_plate_strips(n, n, DX)
+    n_int = n * n
+    top = np.zeros(len(x), bool)
+    bot = np.zeros(len(x), bool)
+    top[n_int:n_int + 3 * n] = True
+    bot[n_int + 3 * n:] = True
+    return x, y, n_int, top, bot
+
+
+def base_increment(x, y, n_int, top):
+    """Uniform-stretch increment between the strips: u_y = du * (y - c) / (y_top - c)
+    with c the plate centre and y_top the mean height of the top strip rows."""
+    cy = y[:n_int].mean()
+    ys = y[top].mean() - cy
+    b = np.zeros((2, len(x)))
+    b[1] = DU * np.clip((y - cy) / ys, -1.0, 1.0)
+    return b
+
+
+def normalized_positions(x, y, n_int):
+    cx, cy = x[:n_int].mean(), y[:n_int].mean()
+    half = 0.5 * max(np.ptp(x[:n_int]), np.ptp(y[:n_int]), DX)
+    return np.stack([(x - cx) / half, (y - cy) / half])
+
+
+def edge_features(x, y, row_ptr, col, mask, delta):
+    rows = np.repeat(np.arange(len(row_ptr) - 1), np.diff(row_ptr))
+    ef = np.empty((len(col), 3), np.float32)
+    ef[:, 0] = (x[col] - x[rows]) / delta
+    ef[:, 1] = (y[col] - y[rows]) / delta
+    ef[:, 2] = mask
+    return ef
+
+
+def damage(row_ptr, mask):
+    """Unweighted broken-bond fraction per particle."""
+    deg = np.diff(row_ptr)
+    intact = np.add.reduceat(mask.astype(np.float64), row_ptr[:-1]) if len(mask) else np.zeros(len(deg))
+    intact = np.where(deg > 0, intact, 0.0)
+    return np.where(deg > 0, 1.0 - intact / np.maximum(deg, 1), 0.0)
+
+
+def node_features(pos, du_prev, dmg, top, bot):
+    """pos: [2][N] coordinates relative to the plate centre / half side."""
+    N = pos.shape[1]
+    f = np.empty((N, 8), np.float32)
+    f[:, 0:2] = pos.T
+    f[:, 2:4] = (du_prev / DU).T
+    f[:, 4] = dmg
+    f[:, 5] = top
+    f[:, 6] = bot
+    f[:, 7] = 1.0
+    return f
+
+
+# ------------------------------------------------------------ training data
+def generate(specs, n_inc, tol=1e-6, max_iters=400000, block=1024, log=None):
+    """Run the batched ADR on ``specs`` for ``n_inc`` increments and keep every
+    converged state.  Returns one record per specimen with its graph and the
+    states u_0 = 0, u_1 .. u_n_inc (float64 [2][N]) and intact masks."""
+    p = PB.plate_params(DX)
+    recs = []
+    t0 = time.perf_counter()
+    with PB.Batch(p, specs) as b:
+        for s, (n, _, _, _) in enumerate(specs):
+            rp, col, mask = b.csr(s)
+            x, y, n_int, top, bot = plate_layout(n)
+            recs.append(dict(n=n, n_int=n_int, rp=rp, col=col, x=x, y=y, top=top, bot=bot,
+                             u=[np.zeros((2, len(x)))], mask=[mask]))
+        iters = []
+        for m in range(n_inc):
+            its, res, ms = b.increment(DU, tol, max_iters, block=block)
+            iters.append(its)
+            for s, r in enumerate(recs):
+                r["u"].append(b.get(s, "U"))
+                r["mask"].append(b.csr(s)[2])
+            if log:
+                log(f"[data] increment {m}: iterations median {int(np.median(its))} max {its.max()}, "
+                    f"{ms:.0f} ms, max residual {res.max():.2e}")
+    return recs, np.array(iters), time.perf_counter() - t0
+
+
+def uses_network(k, mask_before, mask_prev):
+    """The network predicts increment k (0-based) when there is no previous
+    increment or when bonds broke during the previous one."""
+    return k == 0 or mask_prev is None or not np.array_equal(mask_before, mask_prev)
+
+
+def samples(rec, delta):
+    """(node features, edge features, target, weight) of the increments the
+    network predicts (see uses_network)."""
+    us, masks = rec["u"], rec["mask"]
+    free = np.zeros(len(rec["x"]), np.float32)
+    free[: rec["n_int"]] = 1.0
+    pos = normalized_positions(rec["x"], rec["y"], rec["n_int"])
+    base = base_increment(rec["x"], rec["y"], rec["n_int"], rec["top"])
+    for k in range(1, len(us)):
+        if not uses_network(k - 1, masks[k - 1], masks[k - 2] if k >= 2 else None):
+            continue
+        prev = us[k - 1] - us[k - 2] if k >= 2 else np.zeros_like(us[0])
+        mask = masks[k - 1]
+        x = node_features(pos, prev, damage(rec["rp"], mask), rec["top"], rec["bot"])
+        ef = edge_features(rec["x"], rec["y"], rec["rp"], rec["col"], mask, delta)
+        t = ((us[k] - us[k - 1] - base) / DU).T.astype(np.float32)
+        yield x, ef, t, free
+
+
+def train(net, recs, delta, epochs, lr=1e-3, lr_end=1e-4, seed=0, log=None):
+    """Adam over all (specimen, increment) samples; the learning rate decays
+    geometrically from lr to lr_end.  Returns the mean loss of each epoch."""
+    rng = np.random.default_rng(seed)
+    n_samples = sum(1 for r in recs for _ in samples(r, delta))
+    total = epochs * n_samples
+    step = 0
+    hist = []
+    for ep in range(epochs):
+        losses = []
+        for s in rng.permutation(len(recs)):
+            r = recs[s]
+            net.set_graph(r["rp"], r["col"])
+            for x, ef, t, w in samples(r, delta):
+                rate = lr * (lr_end / lr) ** (step / max(total - 1, 1))
+                net.set_features(x, ef, t, w)
+                losses.append(net.train_step(rate))
+                step += 1
+        hist.append(float(np.mean(losses)))
+        if log:
+            log(f"[train] epoch {ep}: mean loss {hist[-1]:.4e} over {len(losses)} samples")
+    return hist
+
+
+# --------------------------------------------------------------- evaluation
+class Plate:
+    """One pre-cracked plate on its own pd_handle, loaded increment by increment."""
+
+    def __init__(self, n, crack_len, angle, device=0):
+        p = PB.plate_params(DX)
+        p.n_top = p.n_bot = 3 * n
+        self.x, self.y, self.n_int, self.top, self.bot = plate_layout(n)
+        self.prob = Problem(p, device)
+        self.prob.set_positions(self.x, self.y, None, n_int=self.n_int)
+        self.prob.build_neighbors()
+        self.prob.init_state(static=True)
+        if crack_len > 0:
+            self.prob.set_crack_2d(crack_len, 0.0, 0.0, angle)
+        self.delta = p.delta
+        self.rp, self.col = self.prob.csr()
+        self.pos = normalized_positions(self.x, self.y, self.n_int)
+        self.base = base_increment(self.x, self.y, self.n_int, self.top)
+        self.du_prev = np.zeros((2, len(self.x)))
+        self.mask_prev = None  # intact mask before the previous increment
+        self.k = 0
+
+    def close(self):
+        self.prob.close()
+
+    def _solve(self, tol, max_iters, probe):
+        it1, r1, ok = self.prob.adr_solve(0, min(probe, max_iters), tol)
+        if ok or it1 < probe:
+            return it1, r1, r1
+        it2, r2, _ = self.prob.adr_solve(probe, max_iters - probe, tol)
+        return it1 + it2, r2, r1
+
+    def step_plain(self, tol=1e-6, max_iters=400000, probe=50):
+        """Plain ADR increment.  Returns (iterations, residual, residual after probe, seconds)."""
+        t = time.perf_counter()
+        u0 = self.prob.get("U")
+        self.prob.strip_displace(DU, -DU)
+        its, r, r_probe = self._solve(tol, max_iters, probe)
+        self.prob.sync()
+        sec = time.perf_counter() - t
+        self.du_prev = self.prob.get("U") - u0
+        self.k += 1
+        return its, r, r_probe, sec
+
+    def step_warm(self, net, r_plain_probe, tol=1e-6, max_iters=400000, probe=50, use_net=True):
+        """Warm-started increment: du_prev, or where uses_network says so the
+        uniform-stretch increment plus (when use_net) the network's
+        correction.  Returns (iterations, residual, seconds, fell_back,
+        predict seconds, used_net)."""
+        t = time.perf_counter()
+        u0 = self.prob.get("U")
+        mask0 = self.prob.fail_mask()
+        guess = u0 + self.du_prev
+        fresh = uses_network(self.k, mask0, self.mask_prev)
+        used_net = use_net and fresh
+        if fresh:
+            guess = u0 + self.base
+        if used_net:
+            x = node_features(self.pos, self.du_prev, damage(self.rp, mask0), self.top, self.bot)
+            net.set_features(x, edge_features(self.x, self.y, self.rp, self.col, mask0, self.delta))
+            guess[1, : self.n_int] += DU * net.forward()[: self.n_int, 1].astype(np.float64)
+        t_pred = time.perf_counter() - t
+        ni, nt = self.n_int, 3 * (len(self.x) - self.n_int) // 6
+        guess[:, ni:] = u0[:, ni:]
+        guess[1, ni:ni + nt] += DU
+        guess[1, ni + nt:] -= DU
+        self.prob.set("U", guess)
+        its, r, _ = self.prob.adr_solve(0, probe, tol)
+        fell_back = False
+        if r > tol and r > r_plain_probe:
+            # the guess is doing worse than plain ADR did after as many
+            # iterations: restart this increment from the plain initial state
+            fell_back = True
+            self.prob.set_fail_mask(mask0)
+            start = u0.copy()
+            start[1, ni:ni + nt] += DU
+            start[1, ni + nt:] -= DU
+            self.prob.set("U", start)
+            it2, r, _ = self._solve(tol, max_iters, probe)
+            its += it2
+        elif r > tol:
+            it2, r, _ = self.prob.adr_solve(probe, max_iters - probe, tol)
+            its += it2
+        self.prob.sync()
+        sec = time.perf_counter() - t
+        self.du_prev = self.prob.get("U") - u0
+        self.mask_prev = mask0
+        self.k += 1
+        return its, r, sec, fell_back, t_pred, used_net
+
+
+def eval_specs(count, n_min=200, n_max=400, seed_base=0x5EED1004):
+    return PB.plate_family(count, seed_base=seed_base, n_min=n_min, n_max=n_max, dx=DX)
+
+
+def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None):
+    """Plain ADR, the warm start without the network (du_prev, uniform stretch)
+    and with it, on the same plates and increments (three handles per plate,
+    each on its own load path).
+    Returns per-increment rows (spec, k, its_plain, its_extrap, its_net, t_plain,
+    t_extrap, t_net, fallback_extrap, fallback_net, used_net)."""
+    rows = []
+    for s, (n, _, clen, ang) in enumerate(specs):
+        plain, extrap, warm = Plate(n, clen, ang), Plate(n, clen, ang), Plate(n, clen, ang)
+        net.set_graph(warm.rp, warm.col)
+        try:
+            for k in range(n_inc):
+                ip, rp_, rprobe, tp = plain.step_plain(tol, probe=probe)
+                ie, re_, te, fe, _, _ = extrap.step_warm(net, rprobe, tol, probe=probe, use_net=False)
+                iw, rw, tw, fw, tpred, used = warm.step_warm(net, rprobe, tol, probe=probe, use_net=True)
+                rows.append((s, k, ip, ie, iw, tp, te, tw, fe, fw, used))
+                if log:
+                    log(f"[eval] plate {s} n={n} inc {k}: iterations plain {ip} no-net {ie} net {iw}  "
+                        f"time {tp*1e3:.1f}/{te*1e3:.1f}/{tw*1e3:.1f} ms (predict {tpred*1e3:.1f} ms)"
+                        f"{' [network]' if used else ''}{' fallback-net' if fw else ''}"
+                        f"{' fallback-extrap' if fe else ''}")
+        finally:
+            plain.close()
+            extrap.close()
+            warm.close()
+    return rows
