@@ -1,0 +1,423 @@
+// k_force_lds.hip -- LDS-staged tile variants of the mixed-precision PMB and
+// LPS passes, and the builder of the per-tile staged particle lists they use.
+//
+// Tiles.  The tiled processing order groups the rows into small spatial
+// tiles (order[tile_start[t] .. tile_start[t+1])).  The staged set of a tile
+// is the sorted set of distinct neighbours of its rows; its particle records
+// are copied into LDS once per pass (four consecutive lanes move one 64-byte
+// record, so the copy is a coalesced stream of whole records), and every bond
+// of the tile then reads its neighbour from LDS through a 16-bit local index
+// lidx[k] instead of gathering 64 bytes from the L2 per bond.  The bond
+// stream shrinks to lidx (2 B) + {|xi|, coefficient} (8 B) + the fail bit.
+//
+// Arithmetic, bond slots, the group reduction and the break decision are the
+// ones of the gathering kernels (mixed_bond.h) with the same bond-to-lane
+// assignment, so both paths produce the same bits.
+#include "mixed_bond.h"
+
+#include <climits>
+
+namespace {
+
+constexpr int kHashSlots = 8192;  // per-tile open-addressing table of the builder
+constexpr int kStageCap = 4096;   // largest staged set the builder handles
+
+// lowbias32 integer hash (C. Wellons, public domain)
+__device__ __forceinline__ uint32_t hash32(uint32_t v)
+{
+    v ^= v >> 16;
+    v *= 0x7feb352dU;
+    v ^= v >> 15;
+    v *= 0x846ca68bU;
+    v ^= v >> 16;
+    return v;
+}
+
+// One workgroup per tile.  Phase 0 counts the distinct neighbours (count[t],
+// -1 when the tile exceeds kStageCap); phase 1 writes the sorted staged list
+// at stage_ptr[t] and the local index of every bond of the tile.
+__global__ __launch_bounds__(1024) void k_stage_tiles(int phase, const int32_t *__restrict__ tile_start,
+                                                      const int32_t *__restrict__ order,
+                                                      const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ col, int32_t *count,
+                                                      const int64_t *__restrict__ stage_ptr, int32_t *stage_idx,
+                                                      uint16_t *lidx)
+{
+    __shared__ int32_t table[kHashSlots];
+    __shared__ int32_t list[kStageCap];
+    __shared__ int n_unique, n_full, n_list;
+    const int t = blockIdx.x;
+    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) table[s] = -1;
+    if (threadIdx.x == 0) {
+        n_unique = 0;
+        n_full = 0;
+        n_list = 0;
+    }
+    __syncthreads();
+    const int r0 = tile_start[t], r1 = tile_start[t + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int r = r0 + wave; r < r1; r += nw) {
+        const int i = order[r];
+        for (int32_t k = rp[i] + lane; k < rp[i + 1]; k += 64) {
+            const int c = col[k];
+            uint32_t h = hash32((uint32_t)c) & (kHashSlots - 1);
+            int probe = 0;
+            for (; probe < kHashSlots; ++probe) {
+                const int old = atomicCAS(&table[h], -1, c);
+                if (old == -1) {
+                    atomicAdd(&n_unique, 1);
+                    break;
+                }
+                if (old == c) break;
+                h = (h + 1) & (kHashSlots - 1);
+            }
+            if (probe == kHashSlots) atomicAdd(&n_full, 1);
+        }
+    }
+    __syncthreads();
+    const int nu = n_unique;
+    const bool fits = n_full == 0 && nu <= kStageCap;
+    if (phase == 0) {
+        if (threadIdx.x == 0) count[t] = fits ? nu : -1;
+        return;
+    }
+    if (!fits) return;  // phase 1 runs only after every tile fitted
+    for (int s = threadIdx.x; s < kHashSlots; s += blockDim.x) {
+        const int v = table[s];
+        if (v >= 0) list[atomicAdd(&n_list, 1)] = v;
+    }
+    int p2 = 1;
+    while (p2 < nu) p2 <<= 1;
+    __syncthreads();
+    for (int s = nu + threadIdx.x; s < p2; s += blockDim.x) list[s] = INT_MAX;
+    __syncthreads();
+    for (int kk = 2; kk <= p2; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int s = threadIdx.x; s < p2; s += blockDim.x) {
+                const int q = s ^ j;
+                if (q > s) {
+                    const bool up = (s & kk) == 0;
+                    const int a = list[s], b = list[q];
+                    if ((a > b) == up) {
+                        list[s] = b;
+                        list[q] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t base = stage_ptr[t];
+    for (int s = threadIdx.x; s < nu; s += blockDim.x) stage_idx[base + s] = list[s];
+    for (int r = r0 + wave; r < r1; r += nw) {
+        const int i = order[r];
+        for (int32_t k = rp[i] + lane; k < rp[i + 1]; k += 64) {
+            const int c = col[k];
+            int lo = 0, hi = nu - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (list[mid] < c)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            lidx[k] = (uint16_t)lo;
+        }
+    }
+}
+
+// Tiles handed to the 8 XCDs in contiguous ranges (gridDim.x is a multiple of 8).
+__device__ __forceinline__ int xcd_tile() { return xcd_block(); }
+
+// Copy the tile's staged records into LDS: 4 lanes per 64-byte record.
+__device__ __forceinline__ void stage_records(double2 *lds, const double *__restrict__ rec,
+                                              const int32_t *__restrict__ stage_idx, int64_t sb, int ns)
+{
+    for (int q = threadIdx.x; q < 4 * ns; q += blockDim.x) {
+        const int j = stage_idx[sb + (q >> 2)];
+        lds[q] = reinterpret_cast<const double2 *>(rec + (size_t)j * kRec)[q & 3];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ RecQ lds_rec(const double2 *lds, int l)
+{
+    const double2 a = lds[4 * l], b = lds[4 * l + 1], d = lds[4 * l + 2], e = lds[4 * l + 3];
+    const float2 qr = __builtin_bit_cast(float2, b.y);
+    return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
+}
+
+__device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_t *__restrict__ lidx,
+                                            const float2 *__restrict__ bnd, const uint32_t *__restrict__ fail,
+                                            bool need_intact)
+{
+    const bool valid = k < ke;
+    const int32_t kk = valid ? k : 0;
+    const int l = lidx[kk];
+    const float2 b = bnd[kk];
+    const uint32_t w = fail[kk >> 5];
+    const bool act = valid && (!need_intact || ((w >> (k & 31)) & 1u));
+    return Slot{valid ? l : 0, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
+}
+
+// Row walk over LDS records; same lane/bond assignment as the gathering walk.
+#define PD_LDS_ROW_WALK(NEED_INTACT, BODY)                                                       \
+    {                                                                                            \
+        const int32_t kb = rp[i], ke = rp[i + 1];                                                \
+        int32_t k = kb + lane;                                                                   \
+        Slot sa = fetch_lslot(k, ke, lidx, bnd, fail, NEED_INTACT);                              \
+        Slot sb = fetch_lslot(k + G, ke, lidx, bnd, fail, NEED_INTACT);                          \
+        while (k < ke) {                                                                         \
+            const RecQ ra = lds_rec(lds, sa.c), rb = lds_rec(lds, sb.c);                         \
+            const Slot na = fetch_lslot(k + 2 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
+            const Slot nb = fetch_lslot(k + 3 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
+            BODY(ra, sa, k);                                                                     \
+            BODY(rb, sb, k + G);                                                                 \
+            sa = na;                                                                             \
+            sb = nb;                                                                             \
+            k += 2 * G;                                                                          \
+        }                                                                                        \
+    }
+
+#define PD_TILE_PROLOGUE                                                                         \
+    extern __shared__ double2 lds[];                                                             \
+    if (stop && *stop) return;                                                                   \
+    const int t = xcd_tile();                                                                    \
+    if (t >= ntiles) return;                                                                     \
+    stage_records(lds, rec, stage_idx, stage_ptr[t], (int)(stage_ptr[t + 1] - stage_ptr[t]));    \
+    const int lane = threadIdx.x & (G - 1);                                                      \
+    const int r1 = tile_start[t + 1];
+
+template <int DIM, int G>
+__global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, int ntiles,
+                                                  const int32_t *__restrict__ tile_start,
+                                                  const int32_t *__restrict__ order,
+                                                  const int64_t *__restrict__ stage_ptr,
+                                                  const int32_t *__restrict__ stage_idx,
+                                                  const int32_t *__restrict__ rp, const uint16_t *__restrict__ lidx,
+                                                  const float2 *__restrict__ bnd, uint32_t *fail,
+                                                  const double *__restrict__ rec, F3 f, double s0, float s0f,
+                                                  float win, double ymin)
+{
+    PD_TILE_PROLOGUE
+    for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
+        const int i = order[r];
+        const Rec ri = load_rec(rec, i).r;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+#define PMB_BODY(R, S, K)                                                                        \
+    {                                                                                            \
+        _Pragma("clang fp contract(fast)")                                                       \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const float s = g.num * frcp(S.L0 * (g.L + S.L0));                                       \
+        const float tb = masked(S.coef * s * g.rL, S.mask);                                      \
+        ax += (double)(tb * g.dx);                                                               \
+        ay += (double)(tb * g.dy);                                                               \
+        if (DIM == 3) az += (double)(tb * g.dz);                                                 \
+        check_break<DIM, false>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail);                     \
+    }
+        PD_LDS_ROW_WALK(true, PMB_BODY)
+#undef PMB_BODY
+        const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
+        const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
+        if (lane == 0) {
+            f.p[0][i] = sx;
+            f.p[1][i] = sy;
+            if (DIM == 3) f.p[2][i] = sz;
+        }
+    }
+}
+
+template <int DIM, int G>
+__global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ stop, int ntiles,
+                                                        const int32_t *__restrict__ tile_start,
+                                                        const int32_t *__restrict__ order,
+                                                        const int64_t *__restrict__ stage_ptr,
+                                                        const int32_t *__restrict__ stage_idx,
+                                                        const int32_t *__restrict__ rp,
+                                                        const uint16_t *__restrict__ lidx,
+                                                        const float2 *__restrict__ bnd,
+                                                        const uint32_t *__restrict__ fail, double *rec,
+                                                        const float *__restrict__ minv32, float ctheta,
+                                                        int skip_broken, float *theta32)
+{
+    PD_TILE_PROLOGUE
+    const bool need_intact = skip_broken != 0;
+    // theta/m of this pass is written into the records after every tile has
+    // staged them: the x0 half (doubles 0..3) is read here, the writes go to
+    // double 3 of rows this tile owns, and the next pass reads them.
+    for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
+        const int i = order[r];
+        const Rec ri = load_rec(rec, i).r;
+        double acc = 0.0;
+#define THETA_BODY(R, S, K)                                                                      \
+    {                                                                                            \
+        _Pragma("clang fp contract(fast)")                                                       \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const float e = g.num * frcp(g.L + S.L0); /* |xi+eta| - |xi| */                          \
+        acc += (double)masked(S.coef * S.L0 * e, S.mask);                                        \
+    }
+        PD_LDS_ROW_WALK(need_intact, THETA_BODY)
+#undef THETA_BODY
+        const float th = group_sum<G>(acc);
+        if (lane == 0) {
+            const float rm = minv32[i];
+            const float theta = ctheta * rm * th;
+            theta32[i] = theta;
+            *reinterpret_cast<float2 *>(rec + (size_t)i * kRec + 3) = make_float2(theta * rm, rm);
+        }
+    }
+}
+
+template <int DIM, int G>
+__global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ stop, int ntiles,
+                                                        const int32_t *__restrict__ tile_start,
+                                                        const int32_t *__restrict__ order,
+                                                        const int64_t *__restrict__ stage_ptr,
+                                                        const int32_t *__restrict__ stage_idx,
+                                                        const int32_t *__restrict__ rp,
+                                                        const uint16_t *__restrict__ lidx,
+                                                        const float2 *__restrict__ bnd, uint32_t *fail,
+                                                        const double *__restrict__ rec, float A, float B,
+                                                        double s0, float s0f, float win, double ymin, F3 f)
+{
+    PD_TILE_PROLOGUE
+    for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
+        const int i = order[r];
+        const RecQ qi = load_rec(rec, i);
+        const Rec &ri = qi.r;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+#define LPSF_BODY(R, S, K)                                                                       \
+    {                                                                                            \
+        _Pragma("clang fp contract(fast)")                                                       \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const float e = g.num * frcp(g.L + S.L0);                                                \
+        const float s = e * frcp(S.L0);                                                          \
+        const float tt = masked(S.coef * (A * S.L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        ax += (double)(tt * g.dx);                                                               \
+        ay += (double)(tt * g.dy);                                                               \
+        if (DIM == 3) az += (double)(tt * g.dz);                                                 \
+        check_break<DIM, true>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail);                      \
+    }
+        PD_LDS_ROW_WALK(true, LPSF_BODY)
+#undef LPSF_BODY
+        const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
+        const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
+        if (lane == 0) {
+            f.p[0][i] = sx;
+            f.p[1][i] = sy;
+            if (DIM == 3) f.p[2][i] = sz;
+        }
+    }
+}
+
+inline unsigned tiles_grid(int ntiles) { return (unsigned)((ntiles + 7) / 8 * 8); }
+
+template <class K>
+hipError_t lds_attr(K kernel, size_t bytes)
+{
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+#define PD_DISPATCH_LDS(FN, DIM, G, ...)        \
+    switch (G) {                                \
+    case 8: return FN<DIM, 8>(__VA_ARGS__);     \
+    case 16: return FN<DIM, 16>(__VA_ARGS__);   \
+    case 32: return FN<DIM, 32>(__VA_ARGS__);   \
+    case 64: return FN<DIM, 64>(__VA_ARGS__);   \
+    default: return hipErrorInvalidValue;       \
+    }
+
+template <int DIM, int G>
+hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
+                     const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window,
+                     double ymin)
+{
+    const size_t bytes = (size_t)T.max_staged * 64;
+    hipError_t e = lds_attr(k_pmb_lds<DIM, G>, bytes);
+    if (e != hipSuccess) return e;
+    k_pmb_lds<DIM, G><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order, T.stage_ptr,
+                                                                    T.stage_idx, rp, T.lidx, bnd, fail, rec, f, s0,
+                                                                    (float)s0, (float)(window * s0), ymin);
+    return hipGetLastError();
+}
+
+template <int DIM, int G>
+hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
+                       const float2 *bnd, const uint32_t *fail, double *rec, const float *minv32, float ctheta,
+                       int skip_broken, float *theta32)
+{
+    const size_t bytes = (size_t)T.max_staged * 64;
+    hipError_t e = lds_attr(k_lps_theta_lds<DIM, G>, bytes);
+    if (e != hipSuccess) return e;
+    k_lps_theta_lds<DIM, G><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order,
+                                                                          T.stage_ptr, T.stage_idx, rp, T.lidx, bnd,
+                                                                          fail, rec, minv32, ctheta, skip_broken,
+                                                                          theta32);
+    return hipGetLastError();
+}
+
+template <int DIM, int G>
+hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
+                      const float2 *bnd, uint32_t *fail, const double *rec, float A, float B, double s0,
+                      double window, double ymin, F3 f)
+{
+    const size_t bytes = (size_t)T.max_staged * 64;
+    hipError_t e = lds_attr(k_lps_force_lds<DIM, G>, bytes);
+    if (e != hipSuccess) return e;
+    k_lps_force_lds<DIM, G><<<tiles_grid(T.ntiles_int), T.block, bytes, s>>>(
+        stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0,
+        (float)s0, (float)(window * s0), ymin, f);
+    return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_t *tile_start, const int32_t *order,
+                              const int32_t *rp, const int32_t *col, int32_t *count, const int64_t *stage_ptr,
+                              int32_t *stage_idx, uint16_t *lidx)
+{
+    if (ntiles <= 0) return hipSuccess;
+    k_stage_tiles<<<ntiles, 1024, 0, s>>>(phase, tile_start, order, rp, col, count, stage_ptr, stage_idx, lidx);
+    return hipGetLastError();
+}
+
+int stage_cap() { return kStageCap; }
+
+hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
+                          const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
+                          double window, double ymin)
+{
+    if (T.ntiles <= 0) return hipSuccess;
+    if (dim == 3) {
+        PD_DISPATCH_LDS(pmb_lds_g, 3, group, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+    } else {
+        PD_DISPATCH_LDS(pmb_lds_g, 2, group, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+    }
+}
+
+hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
+                                const int32_t *order, const int32_t *rp, const float2 *bnd, const uint32_t *fail,
+                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32)
+{
+    if (T.ntiles <= 0) return hipSuccess;
+    if (dim == 3) {
+        PD_DISPATCH_LDS(theta_lds_g, 3, group, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta, skip_broken,
+                        theta32)
+    } else {
+        PD_DISPATCH_LDS(theta_lds_g, 2, group, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta, skip_broken,
+                        theta32)
+    }
+}
+
+hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
+                                const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
+                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f)
+{
+    if (T.ntiles_int <= 0) return hipSuccess;
+    if (dim == 3) {
+        PD_DISPATCH_LDS(lpsf_lds_g, 3, group, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
+    } else {
+        PD_DISPATCH_LDS(lpsf_lds_g, 2, group, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
+    }
+}

@@ -27,150 +27,9 @@
 // lies within near_window*s0 of s0 is re-evaluated with the FP64 parity
 // expression (Force_Bond.cu:185-189, Surf_Corr.cu:324-326) before its break
 // is decided, so breaks follow the FP64 stretch of the current state.
-#include "pd_internal.h"
+#include "mixed_bond.h"
 
 namespace {
-
-constexpr int kRec = 8;  // doubles per particle record
-
-__device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v); }
-__device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v); }
-
-// Workgroups are handed to the 8 XCDs round-robin; give XCD k the contiguous
-// block range [k*per, (k+1)*per).  gridDim.x is a multiple of 8.
-__device__ __forceinline__ int xcd_block()
-{
-    const int per = (int)(gridDim.x >> 3);
-    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
-}
-
-struct Rec {
-    double x0, x1, x2, u0, u1, u2;
-};
-
-struct RecQ {
-    Rec r;
-    float q, m;  // theta/m and 1/m
-};
-
-__device__ __forceinline__ RecQ load_rec(const double *__restrict__ rec, int c)
-{
-    const double2 *p = reinterpret_cast<const double2 *>(rec + (size_t)c * kRec);
-    const double2 a = p[0], b = p[1], d = p[2], e = p[3];
-    const float2 qr = __builtin_bit_cast(float2, b.y);
-    return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
-}
-
-__device__ __forceinline__ bool bit_of(uint32_t word, int32_t k) { return (word >> (k & 31)) & 1u; }
-
-__device__ __forceinline__ void bond_break(uint32_t *fail, int32_t k)
-{
-    atomicAnd(&fail[k >> 5], ~(1u << (k & 31)));
-}
-
-// Per-lane sums accumulate the FP32 bond terms in FP64: every FP32 term is
-// exact in FP64, so the running sum carries 29 guard bits (a compensated sum
-// tighter than Neumaier's in FP32).  The group sum reduces those FP64
-// partials over xor shuffles and rounds once to FP32.
-template <int G>
-__device__ __forceinline__ float group_sum(double a)
-{
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    return (float)a;
-}
-
-// Geometry of one bond in FP32 from FP64 differences.  Only this FP32 part
-// may contract a*b+c into FMA; the FP64 parity stretch below keeps the
-// separate roundings of the oracle.
-template <int DIM>
-struct BondGeo {
-    float dx, dy, dz;   // xi + eta
-    float num;          // |xi+eta|^2 - |xi|^2 = eta.(2 xi + eta)
-    float rL;           // 1/|xi+eta|
-    float L;            // |xi+eta|
-};
-
-template <int DIM>
-__device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
-{
-#pragma clang fp contract(fast)
-    const float ex = (float)(rc.x0 - ri.x0), ey = (float)(rc.x1 - ri.x1);
-    const float hx = (float)(rc.u0 - ri.u0), hy = (float)(rc.u1 - ri.u1);
-    BondGeo<DIM> g;
-    g.dx = ex + hx;
-    g.dy = ey + hy;
-    float L2 = g.dx * g.dx + g.dy * g.dy;
-    g.num = hx * (2.f * ex + hx) + hy * (2.f * ey + hy);
-    if (DIM == 3) {
-        const float ez = (float)(rc.x2 - ri.x2), hz = (float)(rc.u2 - ri.u2);
-        g.dz = ez + hz;
-        L2 += g.dz * g.dz;
-        g.num += hz * (2.f * ez + hz);
-    } else {
-        g.dz = 0.f;
-    }
-    g.rL = frsq(L2);
-    g.L = L2 * g.rL;
-    return g;
-}
-
-// FP64 stretch with the parity expressions (PMB order (x_c+u_c)-x_i-u_i for
-// every component, LPS-3D order for z when lps3d).
-template <int DIM, bool LPS3Z>
-__device__ __forceinline__ double stretch64(const Rec &ri, const Rec &rc)
-{
-    double d0 = rc.x0 + rc.u0 - ri.x0 - ri.u0, d1 = rc.x1 + rc.u1 - ri.x1 - ri.u1;
-    double a0 = rc.x0 - ri.x0, a1 = rc.x1 - ri.x1;
-    double r2 = d0 * d0 + d1 * d1, q2 = a0 * a0 + a1 * a1;
-    if (DIM == 3) {
-        double d2 = LPS3Z ? rc.x2 - ri.x2 + rc.u2 - ri.u2 : rc.x2 + rc.u2 - ri.x2 - ri.u2;
-        double a2 = rc.x2 - ri.x2;
-        r2 = r2 + d2 * d2;
-        q2 = q2 + a2 * a2;
-    }
-    double id = sqrt(q2);
-    return (sqrt(r2) - id) / id;
-}
-
-// One bond slot of a lane: neighbour index, |xi| and bond coefficient, and a
-// mask that is all ones when the slot holds a bond that acts in this pass.
-// Slots past the end of the row point at the particle itself with a zero
-// mask, so every lane runs the same branch-free arithmetic.
-struct Slot {
-    int c;
-    float L0, coef;
-    uint32_t mask;
-};
-
-__device__ __forceinline__ Slot fetch_slot(int32_t k, int32_t ke, int self, const int32_t *__restrict__ col,
-                                           const float2 *__restrict__ bnd, const uint32_t *__restrict__ fail,
-                                           bool need_intact)
-{
-    const bool valid = k < ke;
-    const int32_t kk = valid ? k : 0;
-    const int c = col[kk];
-    const float2 b = bnd[kk];
-    const uint32_t w = fail[kk >> 5];
-    const bool act = valid && (!need_intact || ((w >> (k & 31)) & 1u));
-    return Slot{valid ? c : self, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
-}
-
-__device__ __forceinline__ float masked(float v, uint32_t mask)
-{
-    return __uint_as_float(__float_as_uint(v) & mask);
-}
-
-template <int DIM, bool LPS3Z>
-__device__ __forceinline__ void check_break(const Rec &ri, const Rec &rc, float s, const Slot &sl, int32_t k,
-                                            float s0f, float win, double s0, double ymin, uint32_t *fail)
-{
-    if (sl.mask && s >= s0f - win) {
-        const bool brk = (s > s0f + win ? true : stretch64<DIM, LPS3Z>(ri, rc) > s0) &&
-                         (DIM == 2 || (ri.x1 >= ymin && rc.x1 >= ymin));
-        if (brk) bond_break(fail, k);
-    }
-}
 
 // Row walk shared by the three kernels: lanes take bonds k, k+G of the row,
 // then k+2G, k+3G, ...; the slots of the next pair are fetched while the

@@ -108,6 +108,22 @@ __global__ void k_tile_keys(int dim, int N, int n_int, CD3 x, Grid g, uint32_t *
     idx[i] = (uint32_t)i;
 }
 
+__global__ void k_tile_flags(int N, const uint32_t *__restrict__ key, uint32_t *flag)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < N) flag[p] = (p == 0 || key[p] != key[p - 1]) ? 1u : 0u;
+}
+
+// inc = inclusive prefix of the flags: position p lies in tile inc[p] - 1
+__global__ void k_tile_starts(int N, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ inc,
+                              int32_t *tile_start)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    if (flag[p]) tile_start[inc[p] - 1] = p;
+    if (p == N - 1) tile_start[inc[p]] = N;
+}
+
 // One 64-lane wave sorts one row in LDS; four rows per 256-thread block.
 template <int NP2>
 __global__ __launch_bounds__(256) void k_row_sort(int N, const int32_t *__restrict__ row_ptr,
@@ -287,13 +303,15 @@ int build_neighbors_impl(pd_handle *h)
     return PD_OK;
 }
 
-int build_tile_order(pd_handle *h, double tile_len)
+int build_tile_order(pd_handle *h, double tile_len, bool want_tiles)
 {
     // Processing order of the throughput kernels: particles grouped by cubic
     // tiles of edge tile_len (stable inside a tile), so the particles that run
     // together on one XCD gather their neighbours from one compact region.
     // Integrated particles (i < N_int) come first, so order[0, N_int) is the
-    // work list of the passes that only visit integrated rows.
+    // work list of the passes that only visit integrated rows.  With
+    // want_tiles the row range of every non-empty tile is recorded in
+    // h->lds.tile_start (tiles of integrated rows first, then the others).
     const int N = h->N, dim = h->P.dim;
     hipStream_t s = h->stream;
     Grid g{{h->lo[0], h->lo[1], h->lo[2]}, tile_len, {1, 1, 1}};
@@ -322,6 +340,27 @@ int build_tile_order(pd_handle *h, double tile_len)
     NB_TRY(tmp.alloc(tb));
     NB_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.as<uint32_t>(), key2.as<uint32_t>(), idx.as<uint32_t>(),
                                               reinterpret_cast<uint32_t *>(h->order), N, 0, end_bit, s));
+    if (want_tiles && N > 0) {
+        // key (unsorted) is free now: reuse it for the flags, idx for their prefix
+        uint32_t *flag = key.as<uint32_t>(), *inc = idx.as<uint32_t>();
+        k_tile_flags<<<(N + 255) / 256, 256, 0, s>>>(N, key2.as<uint32_t>(), flag);
+        NB_TRY(hipGetLastError());
+        size_t sb = 0;
+        NB_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, sb, flag, inc, N, s));
+        DevBuf tmp2;
+        NB_TRY(tmp2.alloc(sb));
+        NB_TRY(hipcub::DeviceScan::InclusiveSum(tmp2.p, sb, flag, inc, N, s));
+        uint32_t nt = 0, nt_int = 0;
+        NB_TRY(hipMemcpyAsync(&nt, inc + (N - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (h->N_int < N) NB_TRY(hipMemcpyAsync(&nt_int, inc + h->N_int, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        NB_TRY(hipStreamSynchronize(s));
+        if (h->N_int >= N) nt_int = nt + 1;  // every row is integrated
+        h->lds.ntiles = (int)nt;
+        h->lds.ntiles_int = (int)nt_int - 1;
+        NB_TRY(hipMalloc(&h->lds.tile_start, sizeof(int32_t) * (nt + 1)));
+        k_tile_starts<<<(N + 255) / 256, 256, 0, s>>>(N, flag, inc, h->lds.tile_start);
+        NB_TRY(hipGetLastError());
+    }
     NB_TRY(hipStreamSynchronize(s));
     return PD_OK;
 }

@@ -49,6 +49,16 @@ int check_handle(pd_handle *h)
 
 #define LAUNCH(expr) PD_TRY_HIP(h, (expr))
 
+void free_lds(pd_handle *h)
+{
+    dfree(h->lds.tile_start);
+    dfree(h->lds.stage_ptr);
+    dfree(h->lds.stage_idx);
+    dfree(h->lds.lidx);
+    h->lds.ntiles = h->lds.ntiles_int = h->lds.max_staged = 0;
+    h->lds_ok = false;
+}
+
 void free_bonds(pd_handle *h)
 {
     dfree(h->col);
@@ -60,6 +70,7 @@ void free_bonds(pd_handle *h)
     dfree(h->w);
     dfree(h->e);
     dfree(h->bnd);
+    free_lds(h);
     h->Nb = 0;
     h->coef_dirty = true;
     h->have_scr = false;
@@ -68,6 +79,7 @@ void free_bonds(pd_handle *h)
 
 void free_particles(pd_handle *h)
 {
+    free_lds(h);
     D3 *sets[] = {&h->x, &h->u, &h->v, &h->a, &h->b, &h->f, &h->vh, &h->vho, &h->fo};
     for (D3 *s : sets) {
         if (s->p[0]) (void)hipFree(s->p[0]);  // one allocation per set
@@ -97,6 +109,45 @@ hipError_t alloc_set(D3 &s, int N)
     return hipMemsetAsync(p, 0, sizeof(double) * 3 * (size_t)N, 0);
 }
 
+// Staged neighbour lists of the LDS tile kernels for the current order
+// (build_tile_order with want_tiles).  PD_E_CAPACITY when a tile's staged
+// set does not fit the builder or one workgroup's LDS.
+int build_lds_tiles(pd_handle *h)
+{
+    LdsTiles &T = h->lds;
+    hipStream_t s = h->stream;
+    if (T.ntiles <= 0 || !h->col) return PD_E_CAPACITY;
+    std::vector<int32_t> cnt((size_t)T.ntiles);
+    int32_t *dcnt = nullptr;
+    LAUNCH(hipMalloc(&dcnt, sizeof(int32_t) * T.ntiles));
+    hipError_t e = launch_stage_tiles(s, 0, T.ntiles, T.tile_start, h->order, h->row_ptr, h->col, dcnt, nullptr,
+                                      nullptr, nullptr);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), dcnt, sizeof(int32_t) * T.ntiles, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dcnt);
+    LAUNCH(e);
+    int lds_bytes = 0;
+    LAUNCH(hipDeviceGetAttribute(&lds_bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+    std::vector<int64_t> ptr((size_t)T.ntiles + 1, 0);
+    int mx = 0;
+    for (int t = 0; t < T.ntiles; ++t) {
+        if (cnt[t] < 0) return PD_E_CAPACITY;
+        mx = std::max(mx, (int)cnt[t]);
+        ptr[t + 1] = ptr[t] + cnt[t];
+    }
+    if ((int64_t)mx * 64 > lds_bytes || mx > stage_cap()) return PD_E_CAPACITY;
+    LAUNCH(hipMalloc(&T.stage_ptr, sizeof(int64_t) * (T.ntiles + 1)));
+    LAUNCH(hipMalloc(&T.stage_idx, sizeof(int32_t) * (ptr[T.ntiles] > 0 ? ptr[T.ntiles] : 1)));
+    LAUNCH(hipMalloc(&T.lidx, sizeof(uint16_t) * (h->Nb > 0 ? h->Nb : 1)));
+    LAUNCH(hipMemcpyAsync(T.stage_ptr, ptr.data(), sizeof(int64_t) * (T.ntiles + 1), hipMemcpyHostToDevice, s));
+    LAUNCH(launch_stage_tiles(s, 1, T.ntiles, T.tile_start, h->order, h->row_ptr, h->col, nullptr, T.stage_ptr,
+                              T.stage_idx, T.lidx));
+    LAUNCH(hipStreamSynchronize(s));
+    T.max_staged = mx > 0 ? mx : 1;  // invalid bond slots read record 0
+    h->lds_ok = true;
+    return PD_OK;
+}
+
 int ensure_coef(pd_handle *h)
 {
     if (!h->coef_dirty) return PD_OK;
@@ -105,13 +156,37 @@ int ensure_coef(pd_handle *h)
     if (P.precision == PD_PREC_MIXED) {
         LAUNCH(dalloc(&h->rec, (size_t)h->N * 8));
         LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec));
-        // tiled processing order: 16 dx tiles in 3D (a tile plus its horizon
-        // halo is ~0.7 MB of records, inside one XCD's 4 MB L2), 64 dx in 2D
-        int tc = h->tile_cells < 0 ? (P.dim == 3 ? 16 : 64) : h->tile_cells;
+        // Tiled processing order.  LDS-staged tiles: 6 dx cubes in 3D (about
+        // 1500 staged records, 96 KB of LDS), 16 dx squares in 2D.  Gathering
+        // kernels: 16 dx tiles in 3D (a tile plus its horizon halo is ~0.7 MB
+        // of records, inside one XCD's 4 MB L2), 64 dx in 2D.
+        free_lds(h);
         dfree(h->order);
+        const bool want_lds = h->lds_mode != 0;
+        int tc = h->tile_cells;
+        if (tc < 0) tc = want_lds ? (P.dim == 3 ? 6 : 16) : (P.dim == 3 ? 16 : 64);
         if (tc > 0) {
-            int rc = build_tile_order(h, tc * P.dx);
+            int rc = build_tile_order(h, tc * P.dx, want_lds);
             if (rc) return rc;
+            if (want_lds) {
+                rc = build_lds_tiles(h);
+                if (rc == PD_E_CAPACITY && h->lds_mode == 1) {
+                    h->err = "a tile's staged neighbour set does not fit the LDS of one workgroup";
+                    return rc;
+                }
+                if (rc && rc != PD_E_CAPACITY) return rc;
+                if (rc == PD_E_CAPACITY) {
+                    // fall back to the gathering kernels and their tile size
+                    free_lds(h);
+                    dfree(h->order);
+                    if (h->tile_cells < 0) tc = P.dim == 3 ? 16 : 64;
+                    rc = build_tile_order(h, tc * P.dx, false);
+                    if (rc) return rc;
+                }
+            }
+        } else if (h->lds_mode == 1) {
+            h->err = "LDS tiles need a tiled order (tile_cells > 0)";
+            return PD_E_INVALID;
         }
     }
     if (P.model == PD_MODEL_PMB) {
@@ -148,6 +223,13 @@ int pick_group(const pd_handle *h, int pass = 0)
     if (avg > 12) return 16;
     if (avg > 6) return 8;
     return 4;
+}
+
+// The LDS kernels run groups of 8..64 lanes.
+int lds_group(const pd_handle *h, int pass = 0)
+{
+    const int g = pick_group(h, pass);
+    return g < 8 ? 8 : g;
 }
 
 // Constants of the mixed LPS passes.  With t = w (A theta xi + B e) / m, the
@@ -198,8 +280,12 @@ int force_launch(pd_handle *h, const int *stop)
                                    cd3(h->x), cd3(h->u), h->f, P.s0, ymin));
         } else {
             LAUNCH(sync_rec_u(h));
-            LAUNCH(launch_pmb_mixed(s, stop, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->bnd,
-                                    h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
+            if (h->lds_ok)
+                LAUNCH(launch_pmb_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, h->fail,
+                                      h->rec, h->f32, P.s0, P.near_window, ymin));
+            else
+                LAUNCH(launch_pmb_mixed(s, stop, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->bnd,
+                                        h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
         }
     } else {
         // theta for i < N (Force_State.cu:339), force for i < N_int (:359)
@@ -215,10 +301,18 @@ int force_launch(pd_handle *h, const int *stop)
             float ctheta, A, B;
             lps_mixed_consts(P, &ctheta, &A, &B);
             LAUNCH(sync_rec_u(h));
-            LAUNCH(launch_lps_theta_mixed(s, stop, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd, h->fail,
-                                          h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
-            LAUNCH(launch_lps_force_mixed(s, stop, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr, h->col, h->bnd,
-                                          h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
+            if (h->lds_ok) {
+                LAUNCH(launch_lps_theta_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
+                                            h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
+                LAUNCH(launch_lps_force_lds(s, stop, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr, h->bnd,
+                                            h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
+            } else {
+                LAUNCH(launch_lps_theta_mixed(s, stop, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd,
+                                              h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
+                LAUNCH(launch_lps_force_mixed(s, stop, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr,
+                                              h->col, h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin,
+                                              h->f32));
+            }
         }
     }
     return PD_OK;
@@ -751,7 +845,12 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
         if (P.precision == PD_PREC_MIXED) LAUNCH(launch_pack_u(h->stream, nullptr, P.dim, h->N, cd3(h->u), h->rec));
         LAUNCH(hipEventRecord(e[0], h->stream));
         if (P.model == PD_MODEL_PMB && P.precision == PD_PREC_MIXED) {
-            LAUNCH(launch_pmb_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->bnd, h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
+            if (h->lds_ok)
+                LAUNCH(launch_pmb_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
+                                      h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
+            else
+                LAUNCH(launch_pmb_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col,
+                                        h->bnd, h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
             LAUNCH(hipEventRecord(e[1], h->stream));
             LAUNCH(hipEventRecord(e[2], h->stream));
         } else if (P.model == PD_MODEL_PMB) {
@@ -771,10 +870,22 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
             const int g = pick_group(h);
             float ctheta, A, B;
             lps_mixed_consts(P, &ctheta, &A, &B);
-            LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd,
-                                          h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
+            if (h->lds_ok)
+                LAUNCH(launch_lps_theta_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr,
+                                            h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
+                                            h->theta32));
+            else
+                LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, h->order, P.dim, g, h->N, h->row_ptr, h->col,
+                                              h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
+                                              h->theta32));
             LAUNCH(hipEventRecord(e[1], h->stream));
-            LAUNCH(launch_lps_force_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr, h->col, h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
+            if (h->lds_ok)
+                LAUNCH(launch_lps_force_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr,
+                                            h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
+            else
+                LAUNCH(launch_lps_force_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h, 1), h->N_int,
+                                              h->row_ptr, h->col, h->bnd, h->fail, h->rec, A, B, P.s0,
+                                              P.near_window, ymin, h->f32));
             LAUNCH(hipEventRecord(e[2], h->stream));
         }
         LAUNCH(hipEventSynchronize(e[2]));
@@ -861,6 +972,29 @@ int pd_set_tile(pd_handle *h, int tile_cells)
     h->tile_cells = tile_cells;
     h->coef_dirty = true;
     drop_graph(h);
+    return PD_OK;
+}
+
+int pd_set_lds(pd_handle *h, int mode)
+{
+    // LDS-staged tiles for the mixed kernels: -1 where they fit (default),
+    // 0 never (gathering kernels), 1 always (error when a tile does not fit).
+    if (!h) return PD_E_INVALID;
+    REQUIRE(mode >= -1 && mode <= 1, "mode must be -1, 0 or 1");
+    h->lds_mode = mode;
+    h->coef_dirty = true;
+    drop_graph(h);
+    return PD_OK;
+}
+
+int pd_lds_info(pd_handle *h, int *active, int *ntiles, int *max_staged)
+{
+    if (!h) return PD_E_INVALID;
+    int rc = ensure_coef(h);
+    if (rc) return rc;
+    if (active) *active = h->lds_ok ? 1 : 0;
+    if (ntiles) *ntiles = h->lds.ntiles;
+    if (max_staged) *max_staged = h->lds.max_staged;
     return PD_OK;
 }
 

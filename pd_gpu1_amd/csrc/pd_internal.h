@@ -32,6 +32,17 @@ struct DevScalars {
     unsigned pad[3];
 };
 
+// Per-tile staged neighbour lists of the LDS kernels (k_force_lds.hip).
+struct LdsTiles {
+    int ntiles = 0, ntiles_int = 0;  // tiles; tiles holding integrated rows (they come first)
+    int max_staged = 0;              // largest staged set (records of LDS per workgroup)
+    int block = 1024;                // threads per workgroup
+    int32_t *tile_start = nullptr;   // [ntiles + 1] row ranges of order[]
+    int64_t *stage_ptr = nullptr;    // [ntiles + 1] ranges of stage_idx
+    int32_t *stage_idx = nullptr;    // staged particle indices, ascending within a tile
+    uint16_t *lidx = nullptr;        // per bond: position of col[k] in its tile's staged list
+};
+
 struct pd_handle {
     pd_params P{};
     int device = 0;
@@ -50,6 +61,9 @@ struct pd_handle {
     bool rec_u_valid = false;     // the u half of every record equals u
     int32_t *order = nullptr;     // mixed mode: tiled processing order (null = index order)
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
+    int lds_mode = -1;            // LDS-staged tiles: -1 where they fit (default), 0 off, 1 required
+    bool lds_ok = false;          // the LDS tile lists are built and in use
+    LdsTiles lds;
     double *react[3] = {nullptr, nullptr, nullptr};
 
     // bonds (CSR)
@@ -104,7 +118,7 @@ struct pd_handle {
 
 // k_neighbor.hip
 int build_neighbors_impl(pd_handle *h);
-int build_tile_order(pd_handle *h, double tile_len);
+int build_tile_order(pd_handle *h, double tile_len, bool want_tiles);
 hipError_t launch_cell_ids(hipStream_t s, int dim, int N, CD3 x, const double *lo, double cs,
                            const int *ncell, uint32_t *cell_of, uint32_t *idx);
 hipError_t launch_cell_bounds(hipStream_t s, int N, const uint32_t *sorted_cell, int ncells,
@@ -158,6 +172,20 @@ hipError_t launch_lps_force_fp64(hipStream_t s, const int *stop, int dim, int N_
 
 // k_force_mixed.hip (particle records: 8 doubles per particle, x0 | u | (theta/m, 1/m))
 hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double *rec);
+// k_force_lds.hip
+int stage_cap();
+hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_t *tile_start, const int32_t *order,
+                              const int32_t *rp, const int32_t *col, int32_t *count, const int64_t *stage_ptr,
+                              int32_t *stage_idx, uint16_t *lidx);
+hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
+                          const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
+                          double window, double ymin);
+hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
+                                const int32_t *order, const int32_t *rp, const float2 *bnd, const uint32_t *fail,
+                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32);
+hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
+                                const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
+                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f);
 hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, double *rec);
 hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
                             const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,

@@ -171,6 +171,15 @@ class Problem:
     def set_pass_group(self, pass_, group):
         self._c(lib().pd_set_pass_group(self._h, pass_, group))
 
+    def set_lds(self, mode):
+        """LDS-staged tiles: -1 where they fit (default), 0 off, 1 required."""
+        self._c(lib().pd_set_lds(self._h, mode))
+
+    def lds_info(self):
+        a, n, m = C.c_int(), C.c_int(), C.c_int()
+        self._c(lib().pd_lds_info(self._h, C.byref(a), C.byref(n), C.byref(m)))
+        return bool(a.value), n.value, m.value
+
     def set_tile(self, tile_cells):
         self._c(lib().pd_set_tile(self._h, tile_cells))
 

@@ -67,6 +67,31 @@ def c0_plate_pmb(scale=1.0, steps=1000):
     return Scenario("c0_plate_pmb", p, x, y, None, nx * ny, steps, nxyz, setup)
 
 
+def lps2d_plate(scale=1.0, steps=1000, precision=None):
+    """2D state-based LPS plate with strips pulled at +-20 m/s (the LPS branch
+    of Solve_Plate_Crack_Grow.cu:100-167, Force_State.cu:309-393): 6 x 6 mm at
+    dx = 0.1 mm (scale 1), t = dx, E = 192 GPa, nu = 0.3, rho = 7850,
+    s0 = 0.04472, dt = 1.3367e-8 s, 2 mm central crack."""
+    dx = 1e-4
+    L = 0.006 * scale
+    p, nxyz, _ = params(2, _lib.PD_MODEL_LPS, 1.3367e-8, dx, thick=1e-4, length=L, height=L, emod=192e9,
+                        pratio=0.3, dens=7850.0, s0=0.04472)
+    if precision is not None:
+        p.precision = precision
+    nx, ny, _ = nxyz
+    p.n_top = p.n_bot = 3 * nx
+    p.strip_velocity = 20.0
+    x, y = gen_plate_strips(nx, ny, dx)
+    clen = 0.002 * scale
+
+    def setup(prob):
+        prob.init_state()
+        prob.set_crack_2d(clen, 0.0, 0.0, 0.0)
+        prob.lps_weights()
+
+    return Scenario("lps2d_plate", p, x, y, None, nx * ny, steps, nxyz, setup)
+
+
 def kw_s0(emod, pratio, delta, G0=22170.0):
     """Critical stretch of 3D PMB from the fracture energy: sqrt(5 G0 / (9 K delta))."""
     K = emod / (3.0 * (1.0 - 2.0 * pratio))

@@ -350,3 +350,57 @@ def test_mixed_records_follow_every_displacement_update():
         c.set_fail_mask(mask)
         c.force()
         assert np.array_equal(c.get("F"), f_a)
+
+
+def _force_with(sc, lds, tile, u, precision=_lib.PD_PREC_MIXED):
+    with sc.make(precision=precision) as prob:
+        prob.set_lds(lds)
+        prob.set_tile(tile)
+        prob.set("U", u)
+        prob.force()
+        info = prob.lds_info()
+        return prob.get("F"), prob.fail_mask(), info, (prob.get("THETA") if sc.p.model == _lib.PD_MODEL_LPS else None)
+
+
+@pytest.mark.parametrize("case", ["lps3d", "pmb3d", "pmb2d_strips", "lps2d"])
+def test_lds_tiles_match_gathering_kernels(case):
+    rng = np.random.default_rng(31)
+    if case == "lps3d":
+        sc = scenarios.c2_lps_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+        u = np.stack([1.1e-3 * sc.x, 3e-4 * sc.y, 2.05e-3 * sc.z])
+    elif case == "pmb3d":
+        sc = scenarios.pmb_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+        u = np.stack([2.05e-3 * sc.x, -4e-4 * sc.y, 1e-3 * sc.z])
+    elif case == "pmb2d_strips":
+        sc = scenarios.c0_plate_pmb(scale=0.6)
+        u = np.stack([4e-3 * sc.x, 4.1e-3 * sc.y])
+    else:
+        sc = scenarios.lps2d_plate(scale=1.0)
+        u = np.stack([4e-3 * sc.x, 4.1e-3 * sc.y])
+    u = u + rng.uniform(-2e-9, 2e-9, u.shape)
+    f0, m0, i0, th0 = _force_with(sc, 0, 6 if sc.p.dim == 3 else 16, u)
+    f1, m1, i1, th1 = _force_with(sc, 1, 6 if sc.p.dim == 3 else 16, u)
+    assert not i0[0] and i1[0] and i1[1] > 1 and 0 < i1[2] <= 2560, (i0, i1)
+    assert np.array_equal(m0, m1)
+    assert (m1 == 0).sum() > 0
+    assert np.array_equal(f0, f1), float(np.max(np.abs(f0 - f1)) / np.max(np.abs(f0)))
+    if th0 is not None:
+        assert np.array_equal(th0, th1)
+
+
+def test_lds_capacity_is_reported_and_default_falls_back():
+    sc = scenarios.c2_lps_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+    u = np.stack([1.1e-3 * sc.x, 3e-4 * sc.y, 2.05e-3 * sc.z])
+    with sc.make() as prob:
+        prob.set_tile(12)  # a 12 dx tile stages ~5000 records: above the builder's 4096
+        prob.set_lds(1)
+        with pytest.raises(pd.PdError):
+            prob.force()
+        prob.set_lds(-1)
+        prob.set("U", u)
+        prob.force()
+        assert prob.lds_info()[0] is False
+        f_fallback = prob.get("F")
+    f_lds, _, info, _ = _force_with(sc, -1, -1, u)
+    assert info[0]
+    assert np.array_equal(f_fallback, f_lds)
