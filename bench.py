@@ -136,6 +136,38 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6):
     return out
 
 
+def pmb_section(steps=50, warmup=5, reps=10):
+    """PMB force on the same 4M-particle block geometry (mixed precision)."""
+    from pd_gpu1_amd import _lib, autotune, scenarios
+
+    sc = scenarios.pmb_block(precision=_lib.PD_PREC_MIXED)
+    prob = sc.make(device=0)
+    shape = autotune.tune(prob)
+    prob.close()
+    prob = sc.make(device=0)
+    autotune.apply(prob, shape)
+    lds_active = prob.lds_info()[0]
+    prob.run_verlet(warmup)
+    prob.sync()
+    t0 = time.perf_counter()
+    prob.run_verlet(steps)
+    prob.sync()
+    wall = time.perf_counter() - t0
+    ms_force, _ = prob.time_force_passes(reps)
+    N, nb = prob.N, prob.num_bonds
+    prob.close()
+    b = pmb_bytes(nb, N)
+    gbs = b / (ms_force * 1e-3) / 1e9
+    kname = ("k_pmb_lds<3,%d>" if lds_active else "k_pmb_mixed<3,%d>") % shape["groups"][0]
+    return {"workload": "3D PMB block 200x200x100 (4M particles, same jittered lattice), mixed precision, "
+                        f"{steps} Verlet steps at dt=2e-8",
+            "bonds": nb, "value": nb * steps / wall, "unit": "bond-updates/s", "ms_per_step": wall * 1e3 / steps,
+            "force_bond_updates_per_s": nb / (ms_force * 1e-3), "groups": shape["groups"],
+            "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": b,
+                         "ms_per_launch": ms_force, "traffic": load_traffic(kname.split("<")[0])}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +176,7 @@ def main():
     ap.add_argument("--reps", type=int, default=20, help="launches per pass for the kernel timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adr", action="store_true", help="skip the ADR / warm-start measurements")
+    ap.add_argument("--no-pmb", action="store_true", help="skip the PMB block measurement")
     ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter reference)")
@@ -169,14 +202,16 @@ def main():
     prob.sync()
     from pd_gpu1_amd import autotune
     if args.group:
-        groups, tuned = (args.group, args.group), {}
-        autotune.apply(prob, groups)
+        shape = {"lds": True, "tile": -1, "block": 512, "groups": [args.group, args.group], "ms": {}}
+        autotune.apply(prob, shape)
     else:
-        groups, tuned = autotune.tune_group(prob)
+        shape = autotune.tune(prob)
         # the tuning launches advanced the state: start the measured run from rest
         prob.close()
         prob = sc.make(device=0)
-        autotune.apply(prob, groups)
+        autotune.apply(prob, shape)
+    lds_active, lds_tiles, lds_staged = prob.lds_info()
+    groups = shape["groups"]
     prob.sync()
     t_setup = time.perf_counter() - t_setup
     N, N_int, nb = prob.N, prob.N_int, prob.num_bonds
@@ -199,14 +234,15 @@ def main():
     broken = int((prob.fail_mask() == 0).sum())
     prob.close()
 
+    kname = "k_lps_force_lds<3,%d>" if lds_active else "k_lps_force_mixed<3,%d>"
     roofline = {
         "bound": "hbm",
-        "kernel": "k_lps_force_mixed<3,%d>" % groups[1],
+        "kernel": kname % groups[1],
         "achieved": round(gbs_force, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(gbs_force / HBM_PEAK_GBS, 4),
-        "traffic": load_traffic("lps_force_mixed"),
+        "traffic": load_traffic(kname.split("<")[0]),
         "bytes_per_launch": b_force,
         "ms_per_launch": ms_force,
         "theta_pass": {"achieved": round(gbs_theta, 1), "frac": round(gbs_theta / HBM_PEAK_GBS, 4),
@@ -230,10 +266,16 @@ def main():
                    "particles": N, "bonds": nb, "parallelism": "single GPU",
                    "force_bond_updates_per_s": nb / ((ms_theta + ms_force) * 1e-3),
                    "groups": list(groups),
-                   "autotune_ms": {str(k): [round(v[0], 3), round(v[1], 3)] for k, v in tuned.items()},
+                   "launch_shape": {"lds_tiles": lds_active, "tiles": lds_tiles, "max_staged_records": lds_staged,
+                                    "tile_cells": shape["tile"], "block": shape["block"]},
+                   "autotune_ms": shape.get("ms", {}),
                    "broken_bonds_after_run": broken},
         "roofline": roofline,
     }
+    if not args.no_pmb:
+        t = time.perf_counter()
+        out["pmb_block"] = pmb_section()
+        print(f"[bench] PMB block {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_adr:
         t = time.perf_counter()
         out["adr"] = adr_section()

@@ -262,7 +262,7 @@ int pd_set_pass_group(pd_handle *h, int pass, int group);
  * reps times; *gbps = 2*bytes*reps/time.  Used to calibrate counters. */
 int pd_stream_copy(int device, int64_t bytes, int reps, double *gbps);
 /* Work order of the mixed kernels: particles grouped in cubic tiles of edge
- * tile_cells*dx (0 = index order, -1 = default: 6 in 3D and 16 in 2D with
+ * tile_cells*dx (0 = index order, -1 = default: 5 in 3D and 16 in 2D with
  * LDS-staged tiles, 16 and 64 without).  Results do not depend on it. */
 int pd_set_tile(pd_handle *h, int tile_cells);
 /* LDS-staged tiles for the mixed kernels: each tile's neighbour records are
@@ -270,6 +270,8 @@ int pd_set_tile(pd_handle *h, int tile_cells);
  * indices.  mode -1 = where every tile fits (default), 0 = never (per-bond
  * gathers from L2), 1 = required (PD_E_CAPACITY otherwise).  Same results. */
 int pd_set_lds(pd_handle *h, int mode);
+/* Threads per workgroup of the LDS tile kernels: 256, 512 (default) or 1024. */
+int pd_set_lds_block(pd_handle *h, int threads);
 /* Whether the LDS tiles are in use, their number and the largest staged set. */
 int pd_lds_info(pd_handle *h, int *active, int *ntiles, int *max_staged);
 int pd_sync(pd_handle *h);

@@ -1,16 +1,25 @@
 """Launch-shape autotuner for the mixed-precision force kernels.
 
-For a built problem it times each candidate group size G (lanes per
-particle: 4 .. 64; G = 64 is one wave per particle) with HIP events on the
-problem's stream and keeps the fastest.  Results are cached per
-(dim, model, mean row length) in memory so repeated problems of one family
-reuse the choice.
+For a built problem it times candidate launch shapes with HIP events on the
+problem's stream (pd_time_force_passes) and keeps the fastest:
+
+* LDS-staged tiles (the default path where tiles fit): the tile edge in
+  units of dx and the workgroup size are chosen first with 8 lanes per row,
+  then the lanes per row (thread-group size G) of each pass;
+* gathering kernels (no LDS tiles): the lanes per row of each pass, from
+  a few lanes up to one whole wave per particle.
+
+Results are cached per (dim, model, precision, mean row length) in memory so
+repeated problems of one family reuse the choice.
 """
 import json
 import os
 
 _CACHE = {}
-CANDIDATES = (8, 16, 32, 64)
+GATHER_GROUPS = (8, 16, 32, 64)
+LDS_GROUPS = (4, 8, 16)
+LDS_TILES = {3: (4, 5, 6), 2: (12, 16, 24)}
+LDS_BLOCKS = (512, 1024)
 
 
 def _key(prob, p):
@@ -18,32 +27,69 @@ def _key(prob, p):
     return (p.dim, p.model, p.precision, int(round(avg)))
 
 
-def tune_group(prob, reps=3, candidates=CANDIDATES, verbose=False):
-    """Pick the fastest group size of each force pass of `prob` (a
-    pd_gpu1_amd.Problem in mixed precision) and apply them.
+def _time(prob, reps):
+    prob.time_force_passes(1)  # first launch of this shape (code load)
+    return prob.time_force_passes(reps)
 
-    Returns ((g_pass0, g_pass1), {group: (ms_pass0, ms_pass1)})."""
+
+def tune(prob, reps=3, verbose=False):
+    """Pick and apply the fastest launch shape of `prob` (a pd_gpu1_amd
+    Problem in mixed precision).  Returns the shape dict (see apply) with the
+    timings of every candidate ("ms": {label: [ms_pass0, ms_pass1]})."""
     p = prob.params
     key = _key(prob, p)
     if key in _CACHE:
-        best, times = _CACHE[key]
+        shape = _CACHE[key]
+        apply(prob, shape)
+        return shape
+    ms = {}
+    shape = {"lds": False, "tile": -1, "block": 1024, "groups": [0, 0]}
+    prob.set_lds(-1)
+    prob.set_group(0)
+    if prob.lds_info()[0]:
+        best = None
+        for tile in LDS_TILES[p.dim]:
+            for block in LDS_BLOCKS:
+                prob.set_tile(tile)
+                prob.set_lds_block(block)
+                prob.set_group(8)
+                if not prob.lds_info()[0]:
+                    continue
+                t = _time(prob, reps)
+                ms[f"tile{tile}/block{block}/G8"] = [round(t[0], 3), round(t[1], 3)]
+                if verbose:
+                    print(f"[autotune] tile {tile} block {block} G8: {t[0]:.3f} + {t[1]:.3f} ms", flush=True)
+                if best is None or sum(t) < best[0]:
+                    best = (sum(t), tile, block)
+        shape.update(lds=True, tile=best[1], block=best[2])
+        prob.set_tile(shape["tile"])
+        prob.set_lds_block(shape["block"])
+        groups = LDS_GROUPS
     else:
-        times = {}
-        for g in candidates:
-            prob.set_group(g)
-            prob.time_force_passes(1)  # first launch of this shape (code load)
-            times[g] = prob.time_force_passes(reps)
-            if verbose:
-                print(f"[autotune] G={g}: {times[g][0]:.3f} + {times[g][1]:.3f} ms", flush=True)
-        best = (min(times, key=lambda g: times[g][0]), min(times, key=lambda g: times[g][1]))
-        _CACHE[key] = (best, times)
-    apply(prob, best)
-    return best, times
+        prob.set_lds(0)
+        groups = GATHER_GROUPS
+    times = {}
+    for g in groups:
+        prob.set_group(g)
+        times[g] = _time(prob, reps)
+        ms[f"G{g}"] = [round(times[g][0], 3), round(times[g][1], 3)]
+        if verbose:
+            print(f"[autotune] G={g}: {times[g][0]:.3f} + {times[g][1]:.3f} ms", flush=True)
+    shape["groups"] = [min(times, key=lambda g: times[g][0]), min(times, key=lambda g: times[g][1])]
+    shape["ms"] = ms
+    _CACHE[key] = shape
+    apply(prob, shape)
+    return shape
 
 
-def apply(prob, best):
-    prob.set_pass_group(0, best[0])
-    prob.set_pass_group(1, best[1])
+def apply(prob, shape):
+    """Set a tuned shape on a problem of the same family."""
+    prob.set_lds(-1 if shape["lds"] else 0)
+    prob.set_tile(shape["tile"])
+    prob.set_lds_block(shape["block"])
+    prob.set_group(0)
+    prob.set_pass_group(0, shape["groups"][0])
+    prob.set_pass_group(1, shape["groups"][1])
 
 
 def dump(path):
@@ -57,4 +103,4 @@ if __name__ == "__main__":
     from pd_gpu1_amd import scenarios
     sc = scenarios.c2_lps_block()
     with sc.make() as prob:
-        print(tune_group(prob, verbose=True))
+        print(tune(prob, verbose=True))

@@ -20,6 +20,10 @@
 namespace {
 
 constexpr int kHashSlots = 8192;  // per-tile open-addressing table of the builder
+// LDS record stride in 16-byte units: 80 bytes, so that neighbouring records
+// start on different bank groups (a 64-byte stride puts every record on the
+// same 4 of the 64 banks and serialises the 16-byte reads of a wave).
+constexpr int kLdsRec2 = 5;
 constexpr int kStageCap = 4096;   // largest staged set the builder handles
 
 // lowbias32 integer hash (C. Wellons, public domain)
@@ -135,14 +139,15 @@ __device__ __forceinline__ void stage_records(double2 *lds, const double *__rest
 {
     for (int q = threadIdx.x; q < 4 * ns; q += blockDim.x) {
         const int j = stage_idx[sb + (q >> 2)];
-        lds[q] = reinterpret_cast<const double2 *>(rec + (size_t)j * kRec)[q & 3];
+        lds[kLdsRec2 * (q >> 2) + (q & 3)] = reinterpret_cast<const double2 *>(rec + (size_t)j * kRec)[q & 3];
     }
     __syncthreads();
 }
 
 __device__ __forceinline__ RecQ lds_rec(const double2 *lds, int l)
 {
-    const double2 a = lds[4 * l], b = lds[4 * l + 1], d = lds[4 * l + 2], e = lds[4 * l + 3];
+    const double2 *p = lds + kLdsRec2 * l;
+    const double2 a = p[0], b = p[1], d = p[2], e = p[3];
     const float2 qr = __builtin_bit_cast(float2, b.y);
     return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
 }
@@ -321,6 +326,7 @@ hipError_t lds_attr(K kernel, size_t bytes)
 
 #define PD_DISPATCH_LDS(FN, DIM, G, ...)        \
     switch (G) {                                \
+    case 4: return FN<DIM, 4>(__VA_ARGS__);     \
     case 8: return FN<DIM, 8>(__VA_ARGS__);     \
     case 16: return FN<DIM, 16>(__VA_ARGS__);   \
     case 32: return FN<DIM, 32>(__VA_ARGS__);   \
@@ -333,7 +339,7 @@ hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const in
                      const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window,
                      double ymin)
 {
-    const size_t bytes = (size_t)T.max_staged * 64;
+    const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
     hipError_t e = lds_attr(k_pmb_lds<DIM, G>, bytes);
     if (e != hipSuccess) return e;
     k_pmb_lds<DIM, G><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order, T.stage_ptr,
@@ -347,7 +353,7 @@ hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const 
                        const float2 *bnd, const uint32_t *fail, double *rec, const float *minv32, float ctheta,
                        int skip_broken, float *theta32)
 {
-    const size_t bytes = (size_t)T.max_staged * 64;
+    const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
     hipError_t e = lds_attr(k_lps_theta_lds<DIM, G>, bytes);
     if (e != hipSuccess) return e;
     k_lps_theta_lds<DIM, G><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order,
@@ -362,7 +368,7 @@ hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const i
                       const float2 *bnd, uint32_t *fail, const double *rec, float A, float B, double s0,
                       double window, double ymin, F3 f)
 {
-    const size_t bytes = (size_t)T.max_staged * 64;
+    const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
     hipError_t e = lds_attr(k_lps_force_lds<DIM, G>, bytes);
     if (e != hipSuccess) return e;
     k_lps_force_lds<DIM, G><<<tiles_grid(T.ntiles_int), T.block, bytes, s>>>(
@@ -383,6 +389,7 @@ hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_
 }
 
 int stage_cap() { return kStageCap; }
+int lds_record_bytes() { return 16 * kLdsRec2; }
 
 hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
                           const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,

@@ -135,7 +135,7 @@ int build_lds_tiles(pd_handle *h)
         mx = std::max(mx, (int)cnt[t]);
         ptr[t + 1] = ptr[t] + cnt[t];
     }
-    if ((int64_t)mx * 64 > lds_bytes || mx > stage_cap()) return PD_E_CAPACITY;
+    if ((int64_t)mx * lds_record_bytes() > lds_bytes || mx > stage_cap()) return PD_E_CAPACITY;
     LAUNCH(hipMalloc(&T.stage_ptr, sizeof(int64_t) * (T.ntiles + 1)));
     LAUNCH(hipMalloc(&T.stage_idx, sizeof(int32_t) * (ptr[T.ntiles] > 0 ? ptr[T.ntiles] : 1)));
     LAUNCH(hipMalloc(&T.lidx, sizeof(uint16_t) * (h->Nb > 0 ? h->Nb : 1)));
@@ -144,6 +144,7 @@ int build_lds_tiles(pd_handle *h)
                               T.stage_idx, T.lidx));
     LAUNCH(hipStreamSynchronize(s));
     T.max_staged = mx > 0 ? mx : 1;  // invalid bond slots read record 0
+    T.block = h->lds_block;
     h->lds_ok = true;
     return PD_OK;
 }
@@ -156,15 +157,16 @@ int ensure_coef(pd_handle *h)
     if (P.precision == PD_PREC_MIXED) {
         LAUNCH(dalloc(&h->rec, (size_t)h->N * 8));
         LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec));
-        // Tiled processing order.  LDS-staged tiles: 6 dx cubes in 3D (about
-        // 1500 staged records, 96 KB of LDS), 16 dx squares in 2D.  Gathering
-        // kernels: 16 dx tiles in 3D (a tile plus its horizon halo is ~0.7 MB
-        // of records, inside one XCD's 4 MB L2), 64 dx in 2D.
+        // Tiled processing order.  LDS-staged tiles: 5 dx cubes in 3D (about
+        // 900 staged records, 70 KB of LDS: two 512-thread workgroups per
+        // CU), 16 dx squares in 2D.  Gathering kernels: 16 dx tiles in 3D (a
+        // tile plus its horizon halo is ~0.7 MB of records, inside one XCD's
+        // 4 MB L2), 64 dx in 2D.
         free_lds(h);
         dfree(h->order);
         const bool want_lds = h->lds_mode != 0;
         int tc = h->tile_cells;
-        if (tc < 0) tc = want_lds ? (P.dim == 3 ? 6 : 16) : (P.dim == 3 ? 16 : 64);
+        if (tc < 0) tc = want_lds ? (P.dim == 3 ? 5 : 16) : (P.dim == 3 ? 16 : 64);
         if (tc > 0) {
             int rc = build_tile_order(h, tc * P.dx, want_lds);
             if (rc) return rc;
@@ -225,11 +227,13 @@ int pick_group(const pd_handle *h, int pass = 0)
     return 4;
 }
 
-// The LDS kernels run groups of 8..64 lanes.
+// Lanes per row of the LDS kernels: the autotuned or set group, else 8
+// (short per-row walks balance the rows of a tile across the workgroup).
 int lds_group(const pd_handle *h, int pass = 0)
 {
-    const int g = pick_group(h, pass);
-    return g < 8 ? 8 : g;
+    if (pass == 1 && h->group2) return h->group2;
+    if (h->group) return h->group;
+    return 8;
 }
 
 // Constants of the mixed LPS passes.  With t = w (A theta xi + B e) / m, the
@@ -983,6 +987,18 @@ int pd_set_lds(pd_handle *h, int mode)
     REQUIRE(mode >= -1 && mode <= 1, "mode must be -1, 0 or 1");
     h->lds_mode = mode;
     h->coef_dirty = true;
+    drop_graph(h);
+    return PD_OK;
+}
+
+int pd_set_lds_block(pd_handle *h, int threads)
+{
+    // Workgroup size of the LDS tile kernels (one workgroup per tile).
+    if (!h) return PD_E_INVALID;
+    REQUIRE(threads == 256 || threads == 512 || threads == 1024, "threads must be 256, 512 or 1024");
+    if (threads != h->lds_block) h->coef_dirty = true;
+    h->lds_block = threads;
+    h->lds.block = threads;
     drop_graph(h);
     return PD_OK;
 }

@@ -63,6 +63,7 @@ struct pd_handle {
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
     int lds_mode = -1;            // LDS-staged tiles: -1 where they fit (default), 0 off, 1 required
     bool lds_ok = false;          // the LDS tile lists are built and in use
+    int lds_block = 512;          // threads per workgroup of the LDS kernels
     LdsTiles lds;
     double *react[3] = {nullptr, nullptr, nullptr};
 
@@ -174,6 +175,7 @@ hipError_t launch_lps_force_fp64(hipStream_t s, const int *stop, int dim, int N_
 hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double *rec);
 // k_force_lds.hip
 int stage_cap();
+int lds_record_bytes();
 hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_t *tile_start, const int32_t *order,
                               const int32_t *rp, const int32_t *col, int32_t *count, const int64_t *stage_ptr,
                               int32_t *stage_idx, uint16_t *lidx);

@@ -175,6 +175,9 @@ class Problem:
         """LDS-staged tiles: -1 where they fit (default), 0 off, 1 required."""
         self._c(lib().pd_set_lds(self._h, mode))
 
+    def set_lds_block(self, threads):
+        self._c(lib().pd_set_lds_block(self._h, threads))
+
     def lds_info(self):
         a, n, m = C.c_int(), C.c_int(), C.c_int()
         self._c(lib().pd_lds_info(self._h, C.byref(a), C.byref(n), C.byref(m)))

@@ -380,7 +380,7 @@ def test_lds_tiles_match_gathering_kernels(case):
     u = u + rng.uniform(-2e-9, 2e-9, u.shape)
     f0, m0, i0, th0 = _force_with(sc, 0, 6 if sc.p.dim == 3 else 16, u)
     f1, m1, i1, th1 = _force_with(sc, 1, 6 if sc.p.dim == 3 else 16, u)
-    assert not i0[0] and i1[0] and i1[1] > 1 and 0 < i1[2] <= 2560, (i0, i1)
+    assert not i0[0] and i1[0] and i1[1] > 1 and 0 < i1[2] <= 2048, (i0, i1)
     assert np.array_equal(m0, m1)
     assert (m1 == 0).sum() > 0
     assert np.array_equal(f0, f1), float(np.max(np.abs(f0 - f1)) / np.max(np.abs(f0)))
