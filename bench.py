@@ -46,6 +46,16 @@ def lps_pass_bytes(nb, n, n_int, dim=3):
     return theta, force
 
 
+def kernel_stream_bytes(nb, n, dim=3, lds=True, recomputed=True):
+    """Bytes the LDS tile kernels themselves read per pass: per bond the
+    16-bit local index (2 B) and the fail bit (1/8 B), plus the 8-byte
+    coefficient pair when it is streamed rather than recomputed; per particle
+    its own 64-byte record, row_ptr (4 B), the output (4D B) and the staged
+    records (counted in the measured traffic, not here)."""
+    per_bond = (2.0 if lds else 4.0) + 0.125 + (0.0 if recomputed else 8.0)
+    return per_bond * nb + (64 + 4 + 4 * dim) * n
+
+
 def pmb_bytes(nb, n, dim=3):
     """Mixed PMB: col 4 + xi 4 + kappa 4 + fail 1/8 per bond; x0 8D + u 8D + f 4D + row_ptr 4."""
     return 12.125 * nb + (20 * dim + 4) * n
@@ -211,6 +221,7 @@ def main():
         prob = sc.make(device=0)
         autotune.apply(prob, shape)
     lds_active, lds_tiles, lds_staged = prob.lds_info()
+    recomputed = prob.coef_recomputed()
     groups = shape["groups"]
     prob.sync()
     t_setup = time.perf_counter() - t_setup
@@ -245,6 +256,8 @@ def main():
         "traffic": load_traffic(kname.split("<")[0]),
         "bytes_per_launch": b_force,
         "ms_per_launch": ms_force,
+        "kernel_stream_bytes_per_launch": kernel_stream_bytes(nb, N_int, lds=lds_active, recomputed=recomputed),
+        "coefficients_recomputed": recomputed,
         "theta_pass": {"achieved": round(gbs_theta, 1), "frac": round(gbs_theta / HBM_PEAK_GBS, 4),
                        "bytes_per_launch": b_theta, "ms_per_launch": ms_theta},
     }

@@ -270,6 +270,14 @@ int pd_set_tile(pd_handle *h, int tile_cells);
  * indices.  mode -1 = where every tile fits (default), 0 = never (per-bond
  * gathers from L2), 1 = required (PD_E_CAPACITY otherwise).  Same results. */
 int pd_set_lds(pd_handle *h, int mode);
+/* Bond coefficients of the mixed kernels (|xi| and kappa or omega = w fac vol):
+ * -1 = recomputed from the bond's reference length when every stored value
+ * equals that recomputation to FP32 accuracy, which is checked once per
+ * set-up (default; the coefficient stream is then not read), 0 = always
+ * streamed, 1 = recomputed (PD_E_INVALID when they differ). */
+int pd_set_coef_mode(pd_handle *h, int mode);
+/* *recomputed = 1 when the mixed kernels recompute the coefficients. */
+int pd_coef_info(pd_handle *h, int *recomputed);
 /* Threads per workgroup of the LDS tile kernels: 256, 512 (default) or 1024. */
 int pd_set_lds_block(pd_handle *h, int threads);
 /* Whether the LDS tiles are in use, their number and the largest staged set. */

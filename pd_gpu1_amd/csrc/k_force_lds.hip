@@ -152,6 +152,7 @@ __device__ __forceinline__ RecQ lds_rec(const double2 *lds, int l)
     return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
 }
 
+template <bool GC>
 __device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_t *__restrict__ lidx,
                                             const float2 *__restrict__ bnd, const uint32_t *__restrict__ fail,
                                             bool need_intact)
@@ -159,7 +160,7 @@ __device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_
     const bool valid = k < ke;
     const int32_t kk = valid ? k : 0;
     const int l = lidx[kk];
-    const float2 b = bnd[kk];
+    const float2 b = GC ? make_float2(1.f, 0.f) : bnd[kk];
     const uint32_t w = fail[kk >> 5];
     const bool act = valid && (!need_intact || ((w >> (k & 31)) & 1u));
     return Slot{valid ? l : 0, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
@@ -170,12 +171,12 @@ __device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_
     {                                                                                            \
         const int32_t kb = rp[i], ke = rp[i + 1];                                                \
         int32_t k = kb + lane;                                                                   \
-        Slot sa = fetch_lslot(k, ke, lidx, bnd, fail, NEED_INTACT);                              \
-        Slot sb = fetch_lslot(k + G, ke, lidx, bnd, fail, NEED_INTACT);                          \
+        Slot sa = fetch_lslot<GC>(k, ke, lidx, bnd, fail, NEED_INTACT);                              \
+        Slot sb = fetch_lslot<GC>(k + G, ke, lidx, bnd, fail, NEED_INTACT);                          \
         while (k < ke) {                                                                         \
             const RecQ ra = lds_rec(lds, sa.c), rb = lds_rec(lds, sb.c);                         \
-            const Slot na = fetch_lslot(k + 2 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
-            const Slot nb = fetch_lslot(k + 3 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
+            const Slot na = fetch_lslot<GC>(k + 2 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
+            const Slot nb = fetch_lslot<GC>(k + 3 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
             BODY(ra, sa, k);                                                                     \
             BODY(rb, sb, k + G);                                                                 \
             sa = na;                                                                             \
@@ -193,7 +194,7 @@ __device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_
     const int lane = threadIdx.x & (G - 1);                                                      \
     const int r1 = tile_start[t + 1];
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, int ntiles,
                                                   const int32_t *__restrict__ tile_start,
                                                   const int32_t *__restrict__ order,
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
                                                   const int32_t *__restrict__ rp, const uint16_t *__restrict__ lidx,
                                                   const float2 *__restrict__ bnd, uint32_t *fail,
                                                   const double *__restrict__ rec, F3 f, double s0, float s0f,
-                                                  float win, double ymin)
+                                                  float win, double ymin, GeomCoef gc)
 {
     PD_TILE_PROLOGUE
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
@@ -213,8 +214,10 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        const float s = g.num * frcp(S.L0 * (g.L + S.L0));                                       \
-        const float tb = masked(S.coef * s * g.rL, S.mask);                                      \
+        float L0 = S.L0, cf = S.coef;                                                            \
+        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        const float s = g.num * frcp(L0 * (g.L + L0));                                           \
+        const float tb = masked(cf * s * g.rL, S.mask);                                      \
         ax += (double)(tb * g.dx);                                                               \
         ay += (double)(tb * g.dy);                                                               \
         if (DIM == 3) az += (double)(tb * g.dz);                                                 \
@@ -232,7 +235,7 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
     }
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ stop, int ntiles,
                                                         const int32_t *__restrict__ tile_start,
                                                         const int32_t *__restrict__ order,
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
                                                         const float2 *__restrict__ bnd,
                                                         const uint32_t *__restrict__ fail, double *rec,
                                                         const float *__restrict__ minv32, float ctheta,
-                                                        int skip_broken, float *theta32)
+                                                        int skip_broken, float *theta32, GeomCoef gc)
 {
     PD_TILE_PROLOGUE
     const bool need_intact = skip_broken != 0;
@@ -258,8 +261,10 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        const float e = g.num * frcp(g.L + S.L0); /* |xi+eta| - |xi| */                          \
-        acc += (double)masked(S.coef * S.L0 * e, S.mask);                                        \
+        float L0 = S.L0, cf = S.coef;                                                            \
+        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        const float e = g.num * frcp(g.L + L0); /* |xi+eta| - |xi| */                            \
+        acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
         PD_LDS_ROW_WALK(need_intact, THETA_BODY)
 #undef THETA_BODY
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
     }
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ stop, int ntiles,
                                                         const int32_t *__restrict__ tile_start,
                                                         const int32_t *__restrict__ order,
@@ -283,7 +288,7 @@ __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ 
                                                         const uint16_t *__restrict__ lidx,
                                                         const float2 *__restrict__ bnd, uint32_t *fail,
                                                         const double *__restrict__ rec, float A, float B,
-                                                        double s0, float s0f, float win, double ymin, F3 f)
+                                                        double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc)
 {
     PD_TILE_PROLOGUE
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
@@ -295,9 +300,11 @@ __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ 
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        const float e = g.num * frcp(g.L + S.L0);                                                \
-        const float s = e * frcp(S.L0);                                                          \
-        const float tt = masked(S.coef * (A * S.L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        float L0 = S.L0, cf = S.coef;                                                            \
+        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        const float e = g.num * frcp(g.L + L0);                                                  \
+        const float s = e * frcp(L0);                                                            \
+        const float tt = masked(cf * (A * L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
@@ -324,56 +331,64 @@ hipError_t lds_attr(K kernel, size_t bytes)
                                (int)bytes);
 }
 
-#define PD_DISPATCH_LDS(FN, DIM, G, ...)        \
-    switch (G) {                                \
-    case 4: return FN<DIM, 4>(__VA_ARGS__);     \
-    case 8: return FN<DIM, 8>(__VA_ARGS__);     \
-    case 16: return FN<DIM, 16>(__VA_ARGS__);   \
-    case 32: return FN<DIM, 32>(__VA_ARGS__);   \
-    case 64: return FN<DIM, 64>(__VA_ARGS__);   \
-    default: return hipErrorInvalidValue;       \
+#define PD_LSWITCH(FN, DIM, G, GC, GCV, ...)              \
+    switch (G) {                                          \
+    case 4: return FN<DIM, 4, GC>(__VA_ARGS__, GCV);      \
+    case 8: return FN<DIM, 8, GC>(__VA_ARGS__, GCV);      \
+    case 16: return FN<DIM, 16, GC>(__VA_ARGS__, GCV);    \
+    case 32: return FN<DIM, 32, GC>(__VA_ARGS__, GCV);    \
+    case 64: return FN<DIM, 64, GC>(__VA_ARGS__, GCV);    \
+    default: return hipErrorInvalidValue;                 \
     }
 
-template <int DIM, int G>
+// gc != nullptr: recompute the bond coefficients from |xi| (no coefficient stream)
+#define PD_DISPATCH_LDS(FN, DIM, G, GCP, ...)                                \
+    if (GCP) {                                                               \
+        PD_LSWITCH(FN, DIM, G, true, *(GCP), __VA_ARGS__)                    \
+    } else {                                                                 \
+        PD_LSWITCH(FN, DIM, G, false, GeomCoef{}, __VA_ARGS__)               \
+    }
+
+template <int DIM, int G, bool GC>
 hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
                      const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window,
-                     double ymin)
+                     double ymin, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
-    hipError_t e = lds_attr(k_pmb_lds<DIM, G>, bytes);
+    hipError_t e = lds_attr(k_pmb_lds<DIM, G, GC>, bytes);
     if (e != hipSuccess) return e;
-    k_pmb_lds<DIM, G><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order, T.stage_ptr,
+    k_pmb_lds<DIM, G, GC><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order, T.stage_ptr,
                                                                     T.stage_idx, rp, T.lidx, bnd, fail, rec, f, s0,
-                                                                    (float)s0, (float)(window * s0), ymin);
+                                                                    (float)s0, (float)(window * s0), ymin, gc);
     return hipGetLastError();
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
                        const float2 *bnd, const uint32_t *fail, double *rec, const float *minv32, float ctheta,
-                       int skip_broken, float *theta32)
+                       int skip_broken, float *theta32, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
-    hipError_t e = lds_attr(k_lps_theta_lds<DIM, G>, bytes);
+    hipError_t e = lds_attr(k_lps_theta_lds<DIM, G, GC>, bytes);
     if (e != hipSuccess) return e;
-    k_lps_theta_lds<DIM, G><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order,
+    k_lps_theta_lds<DIM, G, GC><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order,
                                                                           T.stage_ptr, T.stage_idx, rp, T.lidx, bnd,
                                                                           fail, rec, minv32, ctheta, skip_broken,
-                                                                          theta32);
+                                                                          theta32, gc);
     return hipGetLastError();
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
                       const float2 *bnd, uint32_t *fail, const double *rec, float A, float B, double s0,
-                      double window, double ymin, F3 f)
+                      double window, double ymin, F3 f, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
-    hipError_t e = lds_attr(k_lps_force_lds<DIM, G>, bytes);
+    hipError_t e = lds_attr(k_lps_force_lds<DIM, G, GC>, bytes);
     if (e != hipSuccess) return e;
-    k_lps_force_lds<DIM, G><<<tiles_grid(T.ntiles_int), T.block, bytes, s>>>(
+    k_lps_force_lds<DIM, G, GC><<<tiles_grid(T.ntiles_int), T.block, bytes, s>>>(
         stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0,
-        (float)s0, (float)(window * s0), ymin, f);
+        (float)s0, (float)(window * s0), ymin, f, gc);
     return hipGetLastError();
 }
 
@@ -393,38 +408,40 @@ int lds_record_bytes() { return 16 * kLdsRec2; }
 
 hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
                           const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
-                          double window, double ymin)
+                          double window, double ymin, const GeomCoef *gc)
 {
     if (T.ntiles <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(pmb_lds_g, 3, group, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_LDS(pmb_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
     } else {
-        PD_DISPATCH_LDS(pmb_lds_g, 2, group, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_LDS(pmb_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
     }
 }
 
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, const uint32_t *fail,
-                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32)
+                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32,
+                                const GeomCoef *gc)
 {
     if (T.ntiles <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(theta_lds_g, 3, group, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta, skip_broken,
-                        theta32)
+        PD_DISPATCH_LDS(theta_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta,
+                        skip_broken, theta32)
     } else {
-        PD_DISPATCH_LDS(theta_lds_g, 2, group, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta, skip_broken,
-                        theta32)
+        PD_DISPATCH_LDS(theta_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta,
+                        skip_broken, theta32)
     }
 }
 
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
-                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f)
+                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f,
+                                const GeomCoef *gc)
 {
     if (T.ntiles_int <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(lpsf_lds_g, 3, group, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
+        PD_DISPATCH_LDS(lpsf_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
     } else {
-        PD_DISPATCH_LDS(lpsf_lds_g, 2, group, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
+        PD_DISPATCH_LDS(lpsf_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
     }
 }

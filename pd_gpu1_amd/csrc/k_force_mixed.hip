@@ -38,12 +38,12 @@ namespace {
     {                                                                                            \
         const int32_t kb = rp[i], ke = rp[i + 1];                                                \
         int32_t k = kb + lane;                                                                   \
-        Slot sa = fetch_slot(k, ke, i, col, bnd, fail, NEED_INTACT);                             \
-        Slot sb = fetch_slot(k + G, ke, i, col, bnd, fail, NEED_INTACT);                         \
+        Slot sa = fetch_slot<GC>(k, ke, i, col, bnd, fail, NEED_INTACT);                             \
+        Slot sb = fetch_slot<GC>(k + G, ke, i, col, bnd, fail, NEED_INTACT);                         \
         while (k < ke) {                                                                         \
             const RecQ ra = load_rec(rec, sa.c), rb = load_rec(rec, sb.c);                       \
-            const Slot na = fetch_slot(k + 2 * G, ke, i, col, bnd, fail, NEED_INTACT);           \
-            const Slot nb = fetch_slot(k + 3 * G, ke, i, col, bnd, fail, NEED_INTACT);           \
+            const Slot na = fetch_slot<GC>(k + 2 * G, ke, i, col, bnd, fail, NEED_INTACT);           \
+            const Slot nb = fetch_slot<GC>(k + 3 * G, ke, i, col, bnd, fail, NEED_INTACT);           \
             BODY(ra, sa, k);                                                                     \
             BODY(rb, sb, k + G);                                                                 \
             sa = na;                                                                             \
@@ -52,11 +52,11 @@ namespace {
         }                                                                                        \
     }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop, const int32_t *__restrict__ order, int n_rows,
                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                    const float2 *__restrict__ bnd, uint32_t *fail, const double *__restrict__ rec, F3 f, double s0,
-                                                   float s0f, float win, double ymin)
+                                                   float s0f, float win, double ymin, GeomCoef gc)
 {
     if (stop && *stop) return;
     const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
@@ -69,8 +69,10 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        const float s = g.num * frcp(S.L0 * (g.L + S.L0));                                       \
-        const float t = masked(S.coef * s * g.rL, S.mask);                                       \
+        float L0 = S.L0, cf = S.coef;                                                            \
+        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        const float s = g.num * frcp(L0 * (g.L + L0));                                           \
+        const float t = masked(cf * s * g.rL, S.mask);                                       \
         ax += (double)(t * g.dx);                                                                \
         ay += (double)(t * g.dy);                                                                \
         if (DIM == 3) az += (double)(t * g.dz);                                                  \
@@ -87,14 +89,14 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
     }
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__ stop, const int32_t *__restrict__ order, int N,
                                                          const int32_t *__restrict__ rp,
                                                          const int32_t *__restrict__ col,
                                                          const float2 *__restrict__ bnd,
                                                          const uint32_t *__restrict__ fail, double *rec,
                                                          const float *__restrict__ minv32, float ctheta,
-                                                         int skip_broken, float *theta32)
+                                                         int skip_broken, float *theta32, GeomCoef gc)
 {
     if (stop && *stop) return;
     const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
@@ -108,8 +110,10 @@ __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        const float e = g.num * frcp(g.L + S.L0); /* |xi+eta| - |xi| */                          \
-        acc += (double)masked(S.coef * S.L0 * e, S.mask);                                        \
+        float L0 = S.L0, cf = S.coef;                                                            \
+        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        const float e = g.num * frcp(g.L + L0); /* |xi+eta| - |xi| */                            \
+        acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
     PD_ROW_WALK(need_intact, THETA_BODY)
 #undef THETA_BODY
@@ -122,13 +126,13 @@ __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__
     }
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__ stop, const int32_t *__restrict__ order, int N_int,
                                                          const int32_t *__restrict__ rp,
                                                          const int32_t *__restrict__ col,
                                                          const float2 *__restrict__ bnd, uint32_t *fail,
                                                          const double *__restrict__ rec, float A, float B,
-                                                         double s0, float s0f, float win, double ymin, F3 f)
+                                                         double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc)
 {
     if (stop && *stop) return;
     const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
@@ -143,9 +147,11 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        const float e = g.num * frcp(g.L + S.L0);                                                \
-        const float s = e * frcp(S.L0);                                                          \
-        const float tt = masked(S.coef * (A * S.L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        float L0 = S.L0, cf = S.coef;                                                            \
+        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        const float e = g.num * frcp(g.L + L0);                                                  \
+        const float s = e * frcp(L0);                                                            \
+        const float tt = masked(cf * (A * L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
@@ -183,52 +189,97 @@ __global__ void k_pack_u(const int *__restrict__ stop, int dim, int N, CD3 u, do
     r[1] = make_double2(dim == 3 ? u.p[2][i] : 0.0, 0.0);
 }
 
+// Does every stored FP32 bond coefficient equal its recomputation from |xi|
+// (geom_coef) to FP32 accuracy?  *ok is cleared at the first bond that does
+// not (user-set coefficients, surface correction, a missing volume correction).
+__global__ void k_geom_check(int dim, int N, const int32_t *__restrict__ rp, const int32_t *__restrict__ col, CD3 x,
+                             const float2 *__restrict__ bnd, GeomCoef gc, int *ok)
+{
+#pragma clang fp contract(fast)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const int c = col[k];
+        const float ex = (float)(x.p[0][c] - x.p[0][i]), ey = (float)(x.p[1][c] - x.p[1][i]);
+        float xi2 = ex * ex + ey * ey;
+        if (dim == 3) {
+            const float ez = (float)(x.p[2][c] - x.p[2][i]);
+            xi2 += ez * ez;
+        }
+        float L0, cf;
+        geom_coef(xi2, gc, L0, cf);
+        const float2 b = bnd[k];
+        if (!(fabsf(L0 - b.x) <= 1e-6f * b.x && fabsf(cf - b.y) <= 1e-5f * fabsf(b.y) + 1e-30f)) {
+            atomicAnd(ok, 0);
+            return;
+        }
+    }
+}
+
 inline int grid8(int64_t threads)
 {
     int64_t b = (threads + 255) / 256;
     return (int)((b + 7) / 8 * 8);
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 hipError_t pmb_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
-                 const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window, double ymin)
+                 const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window, double ymin,
+                 GeomCoef gc)
 {
-    k_pmb_mixed<DIM, G><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, f, s0,
-                                                              (float)s0, (float)(window * s0), ymin);
+    k_pmb_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, f, s0,
+                                                                  (float)s0, (float)(window * s0), ymin, gc);
     return hipGetLastError();
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 hipError_t theta_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
                    const float2 *bnd, const uint32_t *fail, double *rec, const float *minv32, float ctheta, int skip,
-                   float *theta32)
+                   float *theta32, GeomCoef gc)
 {
-    k_lps_theta_mixed<DIM, G><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, minv32,
-                                                                    ctheta, skip, theta32);
+    k_lps_theta_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec,
+                                                                        minv32, ctheta, skip, theta32, gc);
     return hipGetLastError();
 }
 
-template <int DIM, int G>
+template <int DIM, int G, bool GC>
 hipError_t lpsf_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
                   const float2 *bnd, uint32_t *fail, const double *rec, float A, float B, double s0, double window,
-                  double ymin, F3 f)
+                  double ymin, F3 f, GeomCoef gc)
 {
-    k_lps_force_mixed<DIM, G><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, A, B,
-                                                                    s0, (float)s0, (float)(window * s0), ymin, f);
+    k_lps_force_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, A,
+                                                                        B, s0, (float)s0, (float)(window * s0), ymin,
+                                                                        f, gc);
     return hipGetLastError();
 }
 
-#define PD_DISPATCH_G(FN, DIM, G, ...)          \
-    switch (G) {                                \
-    case 4: return FN<DIM, 4>(__VA_ARGS__);     \
-    case 8: return FN<DIM, 8>(__VA_ARGS__);     \
-    case 16: return FN<DIM, 16>(__VA_ARGS__);   \
-    case 32: return FN<DIM, 32>(__VA_ARGS__);   \
-    case 64: return FN<DIM, 64>(__VA_ARGS__);   \
-    default: return hipErrorInvalidValue;       \
+#define PD_GSWITCH(FN, DIM, G, GC, GCV, ...)              \
+    switch (G) {                                          \
+    case 4: return FN<DIM, 4, GC>(__VA_ARGS__, GCV);      \
+    case 8: return FN<DIM, 8, GC>(__VA_ARGS__, GCV);      \
+    case 16: return FN<DIM, 16, GC>(__VA_ARGS__, GCV);    \
+    case 32: return FN<DIM, 32, GC>(__VA_ARGS__, GCV);    \
+    case 64: return FN<DIM, 64, GC>(__VA_ARGS__, GCV);    \
+    default: return hipErrorInvalidValue;                 \
+    }
+
+// gc != nullptr: recompute the bond coefficients from |xi| (no coefficient stream)
+#define PD_DISPATCH_G(FN, DIM, G, GCP, ...)                                  \
+    if (GCP) {                                                               \
+        PD_GSWITCH(FN, DIM, G, true, *(GCP), __VA_ARGS__)                    \
+    } else {                                                                 \
+        PD_GSWITCH(FN, DIM, G, false, GeomCoef{}, __VA_ARGS__)               \
     }
 
 } // namespace
+
+hipError_t launch_geom_check(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col, CD3 x,
+                             const float2 *bnd, GeomCoef gc, int *ok)
+{
+    if (N <= 0) return hipSuccess;
+    k_geom_check<<<(N + 255) / 256, 256, 0, s>>>(dim, N, rp, col, x, bnd, gc, ok);
+    return hipGetLastError();
+}
 
 hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double *rec)
 {
@@ -246,38 +297,40 @@ hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, 
 
 hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
                             const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                            const double *rec, F3 f, double s0, double window, double ymin)
+                            const double *rec, F3 f, double s0, double window, double ymin, const GeomCoef *gc)
 {
     if (n_rows <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_G(pmb_g, 3, group, s, stop, order, n_rows, rp, col, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_G(pmb_g, 3, group, gc, s, stop, order, n_rows, rp, col, bnd, fail, rec, f, s0, window, ymin)
     } else {
-        PD_DISPATCH_G(pmb_g, 2, group, s, stop, order, n_rows, rp, col, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_G(pmb_g, 2, group, gc, s, stop, order, n_rows, rp, col, bnd, fail, rec, f, s0, window, ymin)
     }
 }
 
 hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
-                                  double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32)
+                                  double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32,
+                                  const GeomCoef *gc)
 {
     if (N <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_G(theta_g, 3, group, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, skip_broken,
+        PD_DISPATCH_G(theta_g, 3, group, gc, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, skip_broken,
                       theta32)
     } else {
-        PD_DISPATCH_G(theta_g, 2, group, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, skip_broken,
+        PD_DISPATCH_G(theta_g, 2, group, gc, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, skip_broken,
                       theta32)
     }
 }
 
 hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                                  const double *rec, float A, float B, double s0, double window, double ymin, F3 f)
+                                  const double *rec, float A, float B, double s0, double window, double ymin, F3 f,
+                                  const GeomCoef *gc)
 {
     if (N_int <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_G(lpsf_g, 3, group, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, window, ymin, f)
+        PD_DISPATCH_G(lpsf_g, 3, group, gc, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, window, ymin, f)
     } else {
-        PD_DISPATCH_G(lpsf_g, 2, group, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, window, ymin, f)
+        PD_DISPATCH_G(lpsf_g, 2, group, gc, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, window, ymin, f)
     }
 }

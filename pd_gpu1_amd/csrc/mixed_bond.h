@@ -65,6 +65,7 @@ struct BondGeo {
     float num;          // |xi+eta|^2 - |xi|^2 = eta.(2 xi + eta)
     float rL;           // 1/|xi+eta|
     float L;            // |xi+eta|
+    float xi2;          // |xi|^2
 };
 
 template <int DIM>
@@ -78,17 +79,33 @@ __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
     g.dy = ey + hy;
     float L2 = g.dx * g.dx + g.dy * g.dy;
     g.num = hx * (2.f * ex + hx) + hy * (2.f * ey + hy);
+    g.xi2 = ex * ex + ey * ey;
     if (DIM == 3) {
         const float ez = (float)(rc.x2 - ri.x2), hz = (float)(rc.u2 - ri.u2);
         g.dz = ez + hz;
         L2 += g.dz * g.dz;
         g.num += hz * (2.f * ez + hz);
+        g.xi2 += ez * ez;
     } else {
         g.dz = 0.f;
     }
     g.rL = frsq(L2);
     g.L = L2 * g.rL;
     return g;
+}
+
+// Bond coefficients recomputed from the bond's reference length instead of
+// streamed from HBM (used when every stored coefficient equals this
+// recomputation to FP32 accuracy, which the engine checks once per set-up):
+// the partial-volume factor of Surf_Corr.cu:27-32, and for LPS the influence
+// weight exp(-xi^2/delta^2) of Force_State.cu:317.  coef = scale * fac [* w].
+__device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &L0, float &coef)
+{
+#pragma clang fp contract(fast)
+    L0 = xi2 * frsq(xi2);
+    const float fac = L0 <= gc.lo ? 1.f : (L0 <= gc.hi ? (gc.hi - L0) * gc.inv_dx : 0.f);
+    coef = gc.scale * fac;
+    if (gc.k2 != 0.f) coef *= __builtin_amdgcn_exp2f(gc.k2 * xi2);
 }
 
 // FP64 stretch with the parity expressions (PMB order (x_c+u_c)-x_i-u_i for
@@ -119,6 +136,7 @@ struct Slot {
     uint32_t mask;
 };
 
+template <bool GC>
 __device__ __forceinline__ Slot fetch_slot(int32_t k, int32_t ke, int self, const int32_t *__restrict__ col,
                                            const float2 *__restrict__ bnd, const uint32_t *__restrict__ fail,
                                            bool need_intact)
@@ -126,7 +144,7 @@ __device__ __forceinline__ Slot fetch_slot(int32_t k, int32_t ke, int self, cons
     const bool valid = k < ke;
     const int32_t kk = valid ? k : 0;
     const int c = col[kk];
-    const float2 b = bnd[kk];
+    const float2 b = GC ? make_float2(1.f, 0.f) : bnd[kk];
     const uint32_t w = fail[kk >> 5];
     const bool act = valid && (!need_intact || ((w >> (k & 31)) & 1u));
     return Slot{valid ? c : self, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};

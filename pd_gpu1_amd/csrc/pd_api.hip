@@ -209,6 +209,30 @@ int ensure_coef(pd_handle *h)
             LAUNCH(dalloc(&h->e, h->Nb));
         }
     }
+    h->geom_ok = false;
+    if (P.precision == PD_PREC_MIXED && h->coef_mode != 0 && h->Nb > 0) {
+        GeomCoef gc{};
+        gc.lo = (float)(P.delta - P.dx / 2.0);
+        gc.hi = (float)(P.delta + P.dx / 2.0);
+        gc.inv_dx = (float)(1.0 / P.dx);
+        gc.scale = (float)(P.model == PD_MODEL_PMB ? P.vol * P.bc : P.vol);
+        gc.k2 = P.model == PD_MODEL_LPS ? (float)(-1.4426950408889634 / (P.delta * P.delta)) : 0.f;
+        int *dok = nullptr, ok = 0;
+        LAUNCH(hipMalloc(&dok, sizeof(int)));
+        const int one = 1;
+        hipError_t e = hipMemcpyAsync(dok, &one, sizeof(int), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = launch_geom_check(s, P.dim, h->N, h->row_ptr, h->col, cd3(h->x), h->bnd, gc, dok);
+        if (e == hipSuccess) e = hipMemcpyAsync(&ok, dok, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(dok);
+        LAUNCH(e);
+        if (!ok && h->coef_mode == 1) {
+            h->err = "stored bond coefficients differ from their recomputation from |xi|";
+            return PD_E_INVALID;
+        }
+        h->geom_ok = ok != 0;
+        h->gc = gc;
+    }
     h->coef_dirty = false;
     return PD_OK;
 }
@@ -235,6 +259,8 @@ int lds_group(const pd_handle *h, int pass = 0)
     if (h->group) return h->group;
     return 8;
 }
+
+const GeomCoef *gcp(const pd_handle *h) { return h->geom_ok ? &h->gc : nullptr; }
 
 // Constants of the mixed LPS passes.  With t = w (A theta xi + B e) / m, the
 // pair force (t_i + t_j) vol fac n of Force_State.cu:111-117 (2D: A =
@@ -286,10 +312,10 @@ int force_launch(pd_handle *h, const int *stop)
             LAUNCH(sync_rec_u(h));
             if (h->lds_ok)
                 LAUNCH(launch_pmb_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, h->fail,
-                                      h->rec, h->f32, P.s0, P.near_window, ymin));
+                                      h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
             else
                 LAUNCH(launch_pmb_mixed(s, stop, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->bnd,
-                                        h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
+                                        h->fail, h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
         }
     } else {
         // theta for i < N (Force_State.cu:339), force for i < N_int (:359)
@@ -307,15 +333,15 @@ int force_launch(pd_handle *h, const int *stop)
             LAUNCH(sync_rec_u(h));
             if (h->lds_ok) {
                 LAUNCH(launch_lps_theta_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
-                                            h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
+                                            h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
                 LAUNCH(launch_lps_force_lds(s, stop, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr, h->bnd,
-                                            h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
+                                            h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32, gcp(h)));
             } else {
                 LAUNCH(launch_lps_theta_mixed(s, stop, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd,
-                                              h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32));
+                                              h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
                 LAUNCH(launch_lps_force_mixed(s, stop, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr,
                                               h->col, h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin,
-                                              h->f32));
+                                              h->f32, gcp(h)));
             }
         }
     }
@@ -851,10 +877,10 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
         if (P.model == PD_MODEL_PMB && P.precision == PD_PREC_MIXED) {
             if (h->lds_ok)
                 LAUNCH(launch_pmb_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
-                                      h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
+                                      h->fail, h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
             else
                 LAUNCH(launch_pmb_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col,
-                                        h->bnd, h->fail, h->rec, h->f32, P.s0, P.near_window, ymin));
+                                        h->bnd, h->fail, h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
             LAUNCH(hipEventRecord(e[1], h->stream));
             LAUNCH(hipEventRecord(e[2], h->stream));
         } else if (P.model == PD_MODEL_PMB) {
@@ -877,19 +903,19 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
             if (h->lds_ok)
                 LAUNCH(launch_lps_theta_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr,
                                             h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
-                                            h->theta32));
+                                            h->theta32, gcp(h)));
             else
                 LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, h->order, P.dim, g, h->N, h->row_ptr, h->col,
                                               h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
-                                              h->theta32));
+                                              h->theta32, gcp(h)));
             LAUNCH(hipEventRecord(e[1], h->stream));
             if (h->lds_ok)
                 LAUNCH(launch_lps_force_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr,
-                                            h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32));
+                                            h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32, gcp(h)));
             else
                 LAUNCH(launch_lps_force_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h, 1), h->N_int,
                                               h->row_ptr, h->col, h->bnd, h->fail, h->rec, A, B, P.s0,
-                                              P.near_window, ymin, h->f32));
+                                              P.near_window, ymin, h->f32, gcp(h)));
             LAUNCH(hipEventRecord(e[2], h->stream));
         }
         LAUNCH(hipEventSynchronize(e[2]));
@@ -988,6 +1014,28 @@ int pd_set_lds(pd_handle *h, int mode)
     h->lds_mode = mode;
     h->coef_dirty = true;
     drop_graph(h);
+    return PD_OK;
+}
+
+int pd_set_coef_mode(pd_handle *h, int mode)
+{
+    // Bond coefficients of the mixed kernels: -1 recompute from |xi| when the
+    // stored ones match that (default), 0 always stream them, 1 recompute
+    // (PD_E_INVALID when they do not match).
+    if (!h) return PD_E_INVALID;
+    REQUIRE(mode >= -1 && mode <= 1, "mode must be -1, 0 or 1");
+    h->coef_mode = mode;
+    h->coef_dirty = true;
+    drop_graph(h);
+    return PD_OK;
+}
+
+int pd_coef_info(pd_handle *h, int *recomputed)
+{
+    if (!h || !recomputed) return PD_E_INVALID;
+    int rc = ensure_coef(h);
+    if (rc) return rc;
+    *recomputed = h->geom_ok ? 1 : 0;
     return PD_OK;
 }
 

@@ -32,6 +32,14 @@ struct DevScalars {
     unsigned pad[3];
 };
 
+// Bond coefficients recomputed from |xi| in the mixed kernels (see geom_coef
+// in mixed_bond.h): coef = scale * fac(|xi|) [* 2^(k2 |xi|^2)].
+struct GeomCoef {
+    float lo, hi, inv_dx;  // fac = 1 up to lo, (hi - |xi|)/dx up to hi, 0 beyond
+    float scale;           // vol (LPS omega = w fac vol) or vol*bc (PMB kappa, scr = 1)
+    float k2;              // LPS: -log2(e)/delta^2, so w = 2^(k2 |xi|^2); PMB: 0
+};
+
 // Per-tile staged neighbour lists of the LDS kernels (k_force_lds.hip).
 struct LdsTiles {
     int ntiles = 0, ntiles_int = 0;  // tiles; tiles holding integrated rows (they come first)
@@ -74,6 +82,9 @@ struct pd_handle {
     double *idist = nullptr, *fac = nullptr, *scr = nullptr, *kappa = nullptr;
     double *w = nullptr, *e = nullptr;
     float2 *bnd = nullptr;    // mixed mode: per bond {|xi|, kappa or omega} in FP32
+    int coef_mode = -1;       // mixed mode: -1 recompute from |xi| where that matches bnd, 0 stream bnd, 1 required
+    bool geom_ok = false;     // the kernels recompute the bond coefficients (gc) instead of reading bnd
+    GeomCoef gc{};
     bool coef_dirty = true;   // kappa / bnd need recomputation
     bool have_scr = false;
     bool have_weights = false;
@@ -173,6 +184,8 @@ hipError_t launch_lps_force_fp64(hipStream_t s, const int *stop, int dim, int N_
 
 // k_force_mixed.hip (particle records: 8 doubles per particle, x0 | u | (theta/m, 1/m))
 hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double *rec);
+hipError_t launch_geom_check(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col, CD3 x,
+                             const float2 *bnd, GeomCoef gc, int *ok);
 // k_force_lds.hip
 int stage_cap();
 int lds_record_bytes();
@@ -181,23 +194,23 @@ hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_
                               int32_t *stage_idx, uint16_t *lidx);
 hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
                           const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
-                          double window, double ymin);
+                          double window, double ymin, const GeomCoef *gc);
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, const uint32_t *fail,
-                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32);
+                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
-                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f);
+                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f, const GeomCoef *gc);
 hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, double *rec);
 hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
                             const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                            const double *rec, F3 f, double s0, double window, double ymin);
+                            const double *rec, F3 f, double s0, double window, double ymin, const GeomCoef *gc);
 hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
-                                  double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32);
+                                  double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
 hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                                  const double *rec, float A, float B, double s0, double window, double ymin, F3 f);
+                                  const double *rec, float A, float B, double s0, double window, double ymin, F3 f, const GeomCoef *gc);
 
 // k_integrate.hip
 hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,

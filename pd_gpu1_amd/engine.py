@@ -175,6 +175,15 @@ class Problem:
         """LDS-staged tiles: -1 where they fit (default), 0 off, 1 required."""
         self._c(lib().pd_set_lds(self._h, mode))
 
+    def set_coef_mode(self, mode):
+        """Bond coefficients: -1 recompute from |xi| where they match (default), 0 stream, 1 recompute."""
+        self._c(lib().pd_set_coef_mode(self._h, mode))
+
+    def coef_recomputed(self):
+        r = C.c_int()
+        self._c(lib().pd_coef_info(self._h, C.byref(r)))
+        return bool(r.value)
+
     def set_lds_block(self, threads):
         self._c(lib().pd_set_lds_block(self._h, threads))
 

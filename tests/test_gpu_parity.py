@@ -404,3 +404,44 @@ def test_lds_capacity_is_reported_and_default_falls_back():
     f_lds, _, info, _ = _force_with(sc, -1, -1, u)
     assert info[0]
     assert np.array_equal(f_fallback, f_lds)
+
+
+def _force_coef(sc, mode, u):
+    with sc.make() as prob:
+        prob.set_coef_mode(mode)
+        prob.set("U", u)
+        prob.force()
+        th = prob.get("THETA") if sc.p.model == _lib.PD_MODEL_LPS else None
+        return prob.get("F"), prob.fail_mask(), prob.coef_recomputed(), th
+
+
+@pytest.mark.parametrize("case", ["lps3d", "pmb3d", "pmb2d_strips"])
+def test_recomputed_bond_coefficients_match_streamed(case):
+    rng = np.random.default_rng(41)
+    if case == "lps3d":
+        sc = scenarios.c2_lps_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+        u = np.stack([1.1e-3 * sc.x, 3e-4 * sc.y, 2.05e-3 * sc.z])
+    elif case == "pmb3d":
+        sc = scenarios.pmb_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+        u = np.stack([2.05e-3 * sc.x, -4e-4 * sc.y, 1e-3 * sc.z])
+    else:
+        sc = scenarios.c0_plate_pmb(scale=0.6)
+        sc.p.precision = _lib.PD_PREC_MIXED
+        u = np.stack([4e-3 * sc.x, 4.1e-3 * sc.y])
+    u = u + rng.uniform(-2e-9, 2e-9, u.shape)
+    f0, m0, r0, th0 = _force_coef(sc, 0, u)
+    f1, m1, r1, th1 = _force_coef(sc, -1, u)
+    assert not r0 and r1
+    assert np.array_equal(m0, m1) and (m1 == 0).sum() > 0
+    assert np.max(np.abs(f1 - f0)) <= 1e-5 * np.max(np.abs(f0))
+    if th0 is not None:
+        assert np.max(np.abs(th1 - th0)) <= 1e-5 * np.max(np.abs(th0))
+
+
+def test_surface_corrected_coefficients_are_streamed():
+    sc = scenarios.c1_kalthoff_winkler(steps=10)
+    with sc.make(precision=_lib.PD_PREC_MIXED) as prob:
+        assert prob.coef_recomputed() is False  # scr != 1 on surface bonds
+        prob.set_coef_mode(1)
+        with pytest.raises(pd.PdError):
+            prob.force()
