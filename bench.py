@@ -188,6 +188,9 @@ def main():
     ap.add_argument("--no-adr", action="store_true", help="skip the ADR / warm-start measurements")
     ap.add_argument("--no-pmb", action="store_true", help="skip the PMB block measurement")
     ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
+    ap.add_argument("--shape", default="", metavar="TILE,B0,B1,G0,G1",
+                    help="fixed LDS launch shape instead of autotuning: tile edge (dx), workgroup size and "
+                         "lanes per row of pass 0 and pass 1")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter reference)")
     args = ap.parse_args()
@@ -211,8 +214,12 @@ def main():
     prob = sc.make(device=0)
     prob.sync()
     from pd_gpu1_amd import autotune
-    if args.group:
-        shape = {"lds": True, "tile": -1, "block": 512, "groups": [args.group, args.group], "ms": {}}
+    if args.shape:
+        t, b0, b1, g0, g1 = (int(v) for v in args.shape.split(","))
+        shape = {"lds": True, "tile": t, "block": [b0, b1], "groups": [g0, g1], "ms": {}}
+        autotune.apply(prob, shape)
+    elif args.group:
+        shape = {"lds": True, "tile": -1, "block": [512, 512], "groups": [args.group, args.group], "ms": {}}
         autotune.apply(prob, shape)
     else:
         shape = autotune.tune(prob)

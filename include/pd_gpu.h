@@ -278,8 +278,10 @@ int pd_set_lds(pd_handle *h, int mode);
 int pd_set_coef_mode(pd_handle *h, int mode);
 /* *recomputed = 1 when the mixed kernels recompute the coefficients. */
 int pd_coef_info(pd_handle *h, int *recomputed);
-/* Threads per workgroup of the LDS tile kernels: 256, 512 (default) or 1024. */
+/* Threads per workgroup of the LDS tile kernels: 256, 512 (default) or 1024;
+ * for both passes, or for one (pass 0 = PMB or LPS dilatation, 1 = LPS force). */
 int pd_set_lds_block(pd_handle *h, int threads);
+int pd_set_lds_pass_block(pd_handle *h, int pass, int threads);
 /* Whether the LDS tiles are in use, their number and the largest staged set. */
 int pd_lds_info(pd_handle *h, int *active, int *ntiles, int *max_staged);
 int pd_sync(pd_handle *h);

@@ -43,7 +43,7 @@ def tune(prob, reps=3, verbose=False):
         apply(prob, shape)
         return shape
     ms = {}
-    shape = {"lds": False, "tile": -1, "block": 1024, "groups": [0, 0]}
+    shape = {"lds": False, "tile": -1, "block": [512, 512], "groups": [0, 0]}
     prob.set_lds(-1)
     prob.set_group(0)
     if prob.lds_info()[0]:
@@ -61,9 +61,17 @@ def tune(prob, reps=3, verbose=False):
                     print(f"[autotune] tile {tile} block {block} G8: {t[0]:.3f} + {t[1]:.3f} ms", flush=True)
                 if best is None or sum(t) < best[0]:
                     best = (sum(t), tile, block)
-        shape.update(lds=True, tile=best[1], block=best[2])
-        prob.set_tile(shape["tile"])
-        prob.set_lds_block(shape["block"])
+        # the workgroup size is a launch parameter: pick it per pass at that tile
+        prob.set_tile(best[1])
+        blk = [best[2], best[2]]
+        for p_ in (0, 1):
+            bt = {}
+            for block in LDS_BLOCKS:
+                prob.set_lds_block(block, p_)
+                bt[block] = _time(prob, reps)[p_]
+            blk[p_] = min(bt, key=bt.get)
+            prob.set_lds_block(blk[p_], p_)
+        shape.update(lds=True, tile=best[1], block=blk)
         groups = LDS_GROUPS
     else:
         prob.set_lds(0)
@@ -86,7 +94,8 @@ def apply(prob, shape):
     """Set a tuned shape on a problem of the same family."""
     prob.set_lds(-1 if shape["lds"] else 0)
     prob.set_tile(shape["tile"])
-    prob.set_lds_block(shape["block"])
+    prob.set_lds_block(shape["block"][0], 0)
+    prob.set_lds_block(shape["block"][1], 1)
     prob.set_group(0)
     prob.set_pass_group(0, shape["groups"][0])
     prob.set_pass_group(1, shape["groups"][1])

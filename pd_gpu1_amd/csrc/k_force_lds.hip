@@ -214,8 +214,8 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        float L0 = S.L0, cf = S.coef;                                                            \
-        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
+        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
         const float s = g.num * frcp(L0 * (g.L + L0));                                           \
         const float tb = masked(cf * s * g.rL, S.mask);                                      \
         ax += (double)(tb * g.dx);                                                               \
@@ -261,8 +261,8 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        float L0 = S.L0, cf = S.coef;                                                            \
-        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
+        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
         const float e = g.num * frcp(g.L + L0); /* |xi+eta| - |xi| */                            \
         acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
@@ -300,10 +300,10 @@ __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ 
     {                                                                                            \
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
-        float L0 = S.L0, cf = S.coef;                                                            \
-        if (GC) geom_coef(g.xi2, gc, L0, cf);                                                    \
+        float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
+        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
         const float e = g.num * frcp(g.L + L0);                                                  \
-        const float s = e * frcp(L0);                                                            \
+        const float s = e * (GC ? rL0 : frcp(L0));                                               \
         const float tt = masked(cf * (A * L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
@@ -386,7 +386,7 @@ hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const i
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
     hipError_t e = lds_attr(k_lps_force_lds<DIM, G, GC>, bytes);
     if (e != hipSuccess) return e;
-    k_lps_force_lds<DIM, G, GC><<<tiles_grid(T.ntiles_int), T.block, bytes, s>>>(
+    k_lps_force_lds<DIM, G, GC><<<tiles_grid(T.ntiles_int), T.block2, bytes, s>>>(
         stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0,
         (float)s0, (float)(window * s0), ymin, f, gc);
     return hipGetLastError();

@@ -99,10 +99,11 @@ __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
 // recomputation to FP32 accuracy, which the engine checks once per set-up):
 // the partial-volume factor of Surf_Corr.cu:27-32, and for LPS the influence
 // weight exp(-xi^2/delta^2) of Force_State.cu:317.  coef = scale * fac [* w].
-__device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &L0, float &coef)
+__device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &L0, float &coef, float &rL0)
 {
 #pragma clang fp contract(fast)
-    L0 = xi2 * frsq(xi2);
+    rL0 = frsq(xi2);
+    L0 = xi2 * rL0;
     const float fac = L0 <= gc.lo ? 1.f : (L0 <= gc.hi ? (gc.hi - L0) * gc.inv_dx : 0.f);
     coef = gc.scale * fac;
     if (gc.k2 != 0.f) coef *= __builtin_amdgcn_exp2f(gc.k2 * xi2);

@@ -145,6 +145,7 @@ int build_lds_tiles(pd_handle *h)
     LAUNCH(hipStreamSynchronize(s));
     T.max_staged = mx > 0 ? mx : 1;  // invalid bond slots read record 0
     T.block = h->lds_block;
+    T.block2 = h->lds_block2;
     h->lds_ok = true;
     return PD_OK;
 }
@@ -1039,16 +1040,25 @@ int pd_coef_info(pd_handle *h, int *recomputed)
     return PD_OK;
 }
 
-int pd_set_lds_block(pd_handle *h, int threads)
+int pd_set_lds_pass_block(pd_handle *h, int pass, int threads)
 {
-    // Workgroup size of the LDS tile kernels (one workgroup per tile).
+    // Workgroup size of one pass of the LDS tile kernels (one workgroup per
+    // tile): pass 0 = PMB or LPS dilatation, 1 = LPS force.
     if (!h) return PD_E_INVALID;
+    REQUIRE(pass == 0 || pass == 1, "pass must be 0 or 1");
     REQUIRE(threads == 256 || threads == 512 || threads == 1024, "threads must be 256, 512 or 1024");
-    if (threads != h->lds_block) h->coef_dirty = true;
-    h->lds_block = threads;
-    h->lds.block = threads;
+    if (pass == 0)
+        h->lds_block = h->lds.block = threads;
+    else
+        h->lds_block2 = h->lds.block2 = threads;
     drop_graph(h);
     return PD_OK;
+}
+
+int pd_set_lds_block(pd_handle *h, int threads)
+{
+    int rc = pd_set_lds_pass_block(h, 0, threads);
+    return rc ? rc : pd_set_lds_pass_block(h, 1, threads);
 }
 
 int pd_lds_info(pd_handle *h, int *active, int *ntiles, int *max_staged)

@@ -44,7 +44,8 @@ struct GeomCoef {
 struct LdsTiles {
     int ntiles = 0, ntiles_int = 0;  // tiles; tiles holding integrated rows (they come first)
     int max_staged = 0;              // largest staged set (records of LDS per workgroup)
-    int block = 1024;                // threads per workgroup
+    int block = 512;                 // threads per workgroup (PMB, LPS dilatation)
+    int block2 = 512;                // threads per workgroup of the LPS force pass
     int32_t *tile_start = nullptr;   // [ntiles + 1] row ranges of order[]
     int64_t *stage_ptr = nullptr;    // [ntiles + 1] ranges of stage_idx
     int32_t *stage_idx = nullptr;    // staged particle indices, ascending within a tile
@@ -71,7 +72,8 @@ struct pd_handle {
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
     int lds_mode = -1;            // LDS-staged tiles: -1 where they fit (default), 0 off, 1 required
     bool lds_ok = false;          // the LDS tile lists are built and in use
-    int lds_block = 512;          // threads per workgroup of the LDS kernels
+    int lds_block = 512;          // threads per workgroup of the LDS kernels (pass 0)
+    int lds_block2 = 512;         // ... of the LPS force pass
     LdsTiles lds;
     double *react[3] = {nullptr, nullptr, nullptr};
 

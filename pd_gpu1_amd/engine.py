@@ -184,8 +184,11 @@ class Problem:
         self._c(lib().pd_coef_info(self._h, C.byref(r)))
         return bool(r.value)
 
-    def set_lds_block(self, threads):
-        self._c(lib().pd_set_lds_block(self._h, threads))
+    def set_lds_block(self, threads, pass_=None):
+        if pass_ is None:
+            self._c(lib().pd_set_lds_block(self._h, threads))
+        else:
+            self._c(lib().pd_set_lds_pass_block(self._h, pass_, threads))
 
     def lds_info(self):
         a, n, m = C.c_int(), C.c_int(), C.c_int()
