@@ -104,7 +104,8 @@ __device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &
 #pragma clang fp contract(fast)
     rL0 = frsq(xi2);
     L0 = xi2 * rL0;
-    const float fac = L0 <= gc.lo ? 1.f : (L0 <= gc.hi ? (gc.hi - L0) * gc.inv_dx : 0.f);
+    // 1 up to lo = delta - dx/2, (hi - |xi|)/dx up to hi = lo + dx, 0 beyond
+    const float fac = __builtin_amdgcn_fmed3f((gc.hi - L0) * gc.inv_dx, 0.f, 1.f);
     coef = gc.scale * fac;
     if (gc.k2 != 0.f) coef *= __builtin_amdgcn_exp2f(gc.k2 * xi2);
 }
