@@ -73,7 +73,7 @@ def load_traffic(kernel_key):
         return None
 
 
-def cpu_baseline(sample_dims=(200, 200, 100), evals=8):
+def cpu_baseline(sample_dims=(200, 200, 100), evals=40):
     """FP64 oracle (OpenMP) on a sub-block of the same lattice and physics."""
     import numpy as np
     from oracle.runner import OracleModel

@@ -359,7 +359,7 @@ hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const in
     if (e != hipSuccess) return e;
     k_pmb_lds<DIM, G, GC><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order, T.stage_ptr,
                                                                     T.stage_idx, rp, T.lidx, bnd, fail, rec, f, s0,
-                                                                    (float)s0, (float)(window * s0), ymin, gc);
+                                                                    (float)s0, stretch_window(s0, window), ymin, gc);
     return hipGetLastError();
 }
 
@@ -388,7 +388,7 @@ hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const i
     if (e != hipSuccess) return e;
     k_lps_force_lds<DIM, G, GC><<<tiles_grid(T.ntiles_int), T.block2, bytes, s>>>(
         stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0,
-        (float)s0, (float)(window * s0), ymin, f, gc);
+        (float)s0, stretch_window(s0, window), ymin, f, gc);
     return hipGetLastError();
 }
 

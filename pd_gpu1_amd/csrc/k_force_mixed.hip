@@ -228,7 +228,7 @@ hipError_t pmb_g(hipStream_t s, const int *stop, const int32_t *order, int n, co
                  GeomCoef gc)
 {
     k_pmb_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, f, s0,
-                                                                  (float)s0, (float)(window * s0), ymin, gc);
+                                                                  (float)s0, stretch_window(s0, window), ymin, gc);
     return hipGetLastError();
 }
 
@@ -248,7 +248,7 @@ hipError_t lpsf_g(hipStream_t s, const int *stop, const int32_t *order, int n, c
                   double ymin, F3 f, GeomCoef gc)
 {
     k_lps_force_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, A,
-                                                                        B, s0, (float)s0, (float)(window * s0), ymin,
+                                                                        B, s0, (float)s0, stretch_window(s0, window), ymin,
                                                                         f, gc);
     return hipGetLastError();
 }

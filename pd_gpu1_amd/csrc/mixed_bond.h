@@ -157,6 +157,19 @@ __device__ __forceinline__ float masked(float v, uint32_t mask)
     return __uint_as_float(__float_as_uint(v) & mask);
 }
 
+// Half-width of the band around s0 in which the FP64 stretch decides a break.
+// It is at least near_window*s0 and at least 3e-6, which bounds the error of
+// the FP32 stretch of any bond: with r = |eta|/|xi| and the inputs and the
+// three products of eta.(2 xi + eta) each rounded once, the error is below
+// 2^-24 r (12 + 6 r), and |eta| <= |xi + eta| + |xi| gives r <= 2 + s, so
+// about 1.5e-6 near s0.  Outside the band the FP32 comparison therefore
+// equals the FP64 one for every bond, rotated or not.
+inline float stretch_window(double s0, double near_window)
+{
+    const double w = near_window * s0;
+    return (float)(w > 3.0e-6 ? w : 3.0e-6);
+}
+
 template <int DIM, bool LPS3Z>
 __device__ __forceinline__ void check_break(const Rec &ri, const Rec &rc, float s, const Slot &sl, int32_t k,
                                             float s0f, float win, double s0, double ymin, uint32_t *fail)
