@@ -111,18 +111,23 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6):
     from pd_gpu1_amd import warmstart as W
     from pd_gpu1_amd.mpnn import Mpnn
 
+    from pd_gpu1_amd import autotune
+
     out = {}
     specs = PB.plate_family(n_batch)
+    block, group, tuned = autotune.tune_batch(PB.plate_params(), specs[:32])
     with PB.Batch(PB.plate_params(), specs) as b:
+        b.set_group(group)
         n_part, n_bonds, _ = b.info()
         incs = []
         for _ in range(batch_increments):
-            its, res, ms = b.increment(W.DU, 1e-6, 400000, block=1024)
+            its, res, ms = b.increment(W.DU, 1e-6, 400000, block=block)
             incs.append({"wall_ms": round(ms, 1), "iterations_median": int(np.median(its)),
                          "iterations_max": int(its.max()), "max_residual": float(res.max())})
     out["batched"] = {"workload": f"{n_batch} PMB plates, sides 40..200 (seed 0x5EED0003+k), du=1e-6 m, "
                                   "ADR to relative residual 1e-6, one persistent launch per increment",
-                      "particles": n_part, "bonds": n_bonds, "increments": incs}
+                      "particles": n_part, "bonds": n_bonds, "increments": incs,
+                      "launch_shape": {"block": block, "lanes_per_row": group}, "autotune_ms_per_iteration": tuned}
     wpath = os.path.join(ROOT, "pd_gpu1_amd", "data", "mpnn_warmstart.npy")
     specs = W.eval_specs(n_eval)
     with Mpnn() as net:

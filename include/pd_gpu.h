@@ -221,6 +221,10 @@ int pd_batch_spec_info(pd_batch *b, int spec, int *n, int *n_int, int64_t *nb);
  * iters_out/resid_out[nspec] receive the updates done and the final residual. */
 int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int block, int grid,
                        int32_t *iters_out, double *resid_out, double *ms_out);
+/* Force phase of the batched kernel: 1 thread per particle (default) or
+ * 4, 8, 16, 32 lanes per particle; every choice adds a row's bond terms in
+ * bond order, so results are identical. */
+int pd_batch_set_group(pd_batch *b, int group);
 int pd_batch_get_field(pd_batch *b, int spec, int field, double *out);       /* [2][n] */
 int pd_batch_set_field(pd_batch *b, int spec, int field, const double *in);
 int pd_batch_get_csr(pd_batch *b, int spec, int32_t *row_ptr, int32_t *col, uint32_t *fail_words);

@@ -90,6 +90,26 @@ def tune(prob, reps=3, verbose=False):
     return shape
 
 
+def tune_batch(params, specs, iters=100, blocks=(512, 1024), groups=(1, 4, 8), verbose=False):
+    """Workgroup size and lanes per row (1 = thread per particle, 4/8 = a
+    few lanes per row) of the batched ADR kernel, timed on a fresh batch of
+    `specs` for `iters` iterations per candidate.  Returns (block, group, ms)."""
+    from . import batch as PB
+
+    ms = {}
+    for block in blocks:
+        for g in groups:
+            with PB.Batch(params, specs) as b:
+                b.set_group(g)
+                _, _, t = b.increment(1.0e-6, 0.0, iters, block=block)
+            ms[f"block{block}/G{g}"] = round(t / iters, 4)
+            if verbose:
+                print(f"[autotune] batch block {block} G{g}: {t / iters:.3f} ms/iteration", flush=True)
+    best = min(ms, key=ms.get)
+    block, g = best.split("/")
+    return int(block[5:]), int(g[1:]), ms
+
+
 def apply(prob, shape):
     """Set a tuned shape on a problem of the same family."""
     prob.set_lds(-1 if shape["lds"] else 0)

@@ -94,6 +94,10 @@ class Batch:
         self._c(lib().pd_batch_spec_info(self._h, s, C.byref(n), C.byref(ni), C.byref(nb)))
         return n.value, ni.value, nb.value
 
+    def set_group(self, group):
+        """Lanes per particle of the force phase (1 = thread per particle)."""
+        self._c(lib().pd_batch_set_group(self._h, group))
+
     def increment(self, du, tol, max_iters, block=1024, grid=0):
         its = np.zeros(self.nspec, dtype=np.int32)
         res = np.zeros(self.nspec)

@@ -41,7 +41,25 @@ __device__ __forceinline__ void block_canon4(double p[4], double (*sh)[BT / 64],
     }
 }
 
-template <int BT>
+// Pair force term of bond k of a row at (x0+u0, x1+u1), Force_Bond.cu:185-196
+// operand order; breaks the bond when its stretch exceeds s0.
+__device__ __forceinline__ void bond_term(int64_t k, int c, double x0, double x1, double u0, double u1,
+                                          const double *X, const double *Y, const double *UX, const double *UY,
+                                          const double *kap, const double *idist, double s0, uint32_t *fail,
+                                          double &tx, double &ty)
+{
+    double d0 = X[c] + UX[c] - x0 - u0;
+    double d1 = Y[c] + UY[c] - x1 - u1;
+    double nl = sqrt(d0 * d0 + d1 * d1);
+    double id = idist[k];
+    double sv = (nl - id) / id;
+    double ks = kap[k] * sv;
+    tx = ks * d0 / nl;
+    ty = ks * d1 / nl;
+    if (sv > s0) atomicAnd(&fail[k >> 5], ~(1u << (k & 31)));
+}
+
+template <int BT, int G>
 __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double tol, int max_iters,
                                                   unsigned *queue, int32_t *iters_out, double *resid_out)
 {
@@ -76,24 +94,49 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         double r = 0.0;
         for (;;) {
             // PMB force, every row (Force_Bond.cu:169)
-            for (int i = tid; i < N; i += BT) {
-                const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
-                double fx = 0.0, fy = 0.0;
-                for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-                    if (!intact(fail, k)) continue;
-                    const int c = col[k];
-                    double d0 = X[c] + UX[c] - x0 - u0;
-                    double d1 = Y[c] + UY[c] - x1 - u1;
-                    double nl = sqrt(d0 * d0 + d1 * d1);
-                    double id = idist[k];
-                    double sv = (nl - id) / id;
-                    double ks = kap[k] * sv;
-                    fx += ks * d0 / nl;
-                    fy += ks * d1 / nl;
-                    if (sv > s0) atomicAnd(&fail[k >> 5], ~(1u << (k & 31)));
+            if (G == 1) {
+                for (int i = tid; i < N; i += BT) {
+                    const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
+                    double fx = 0.0, fy = 0.0;
+                    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                        if (!intact(fail, k)) continue;
+                        double tx, ty;
+                        bond_term(k, col[k], x0, x1, u0, u1, X, Y, UX, UY, kap, idist, s0, fail, tx, ty);
+                        fx += tx;
+                        fy += ty;
+                    }
+                    FX[i] = fx;
+                    FY[i] = fy;
                 }
-                FX[i] = fx;
-                FY[i] = fy;
+            } else {
+                const int lane = tid & (G - 1);
+                for (int i = tid / G; i < N; i += BT / G) {  // the whole group takes the row
+                    const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
+                    double fx = 0.0, fy = 0.0;
+                    const int32_t kb = rp[i], ke = rp[i + 1];
+                    for (int32_t k0 = kb; k0 < ke; k0 += G) {
+                        const int32_t k = k0 + lane;
+                        double tx = 0.0, ty = 0.0;
+                        int ok = 0;
+                        if (k < ke && intact(fail, k)) {
+                            ok = 1;
+                            bond_term(k, col[k], x0, x1, u0, u1, X, Y, UX, UY, kap, idist, s0, fail, tx, ty);
+                        }
+                        // add the terms in bond order, as the sequential loop does
+                        const int cnt = ke - k0 < G ? ke - k0 : G;
+                        for (int l = 0; l < cnt; ++l) {
+                            const double sx = __shfl(tx, l, G), sy = __shfl(ty, l, G);
+                            if (__shfl(ok, l, G)) {
+                                fx += sx;
+                                fy += sy;
+                            }
+                        }
+                    }
+                    if (lane == 0) {
+                        FX[i] = fx;
+                        FY[i] = fy;
+                    }
+                }
             }
             __syncthreads();
             // damping and residual terms, lane t owns particles t, t+BT, ...
@@ -171,14 +214,30 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
 
 } // namespace
 
-hipError_t launch_batch_adr(hipStream_t st, int grid, int block, const BatchView &B, double du, double tol,
-                            int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out)
+namespace {
+template <int BT>
+hipError_t batch_g(hipStream_t st, int grid, int group, const BatchView &B, double du, double tol, int max_iters,
+                   unsigned *queue, int32_t *iters_out, double *resid_out)
 {
-    switch (block) {
-    case 256: k_batch_adr<256><<<grid, 256, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
-    case 512: k_batch_adr<512><<<grid, 512, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
-    case 1024: k_batch_adr<1024><<<grid, 1024, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    switch (group) {
+    case 1: k_batch_adr<BT, 1><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    case 4: k_batch_adr<BT, 4><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    case 8: k_batch_adr<BT, 8><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    case 16: k_batch_adr<BT, 16><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    case 32: k_batch_adr<BT, 32><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+} // namespace
+
+hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
+                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out)
+{
+    switch (block) {
+    case 256: return batch_g<256>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out);
+    case 512: return batch_g<512>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out);
+    case 1024: return batch_g<1024>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out);
+    default: return hipErrorInvalidValue;
+    }
 }

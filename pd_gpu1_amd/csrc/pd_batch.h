@@ -21,5 +21,5 @@ struct BatchView {
     double mass, dt, s0;
 };
 
-hipError_t launch_batch_adr(hipStream_t st, int grid, int block, const BatchView &B, double du, double tol,
-                            int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out);
+hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
+                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out);

@@ -27,6 +27,7 @@ struct pd_batch {
     int32_t *d_iters = nullptr;
     double *d_resid = nullptr;
     int last_block = 1024;
+    int group = 1;                // lanes per row of the force phase (1 = thread per particle)
     BatchView view() const
     {
         BatchView v{};
@@ -226,7 +227,8 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     B_TRY(hipEventCreate(&e1));
     B_TRY(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
     B_TRY(hipEventRecord(e0, b->stream));
-    B_TRY(launch_batch_adr(b->stream, grid, block, b->view(), du, tol, max_iters, b->queue, b->d_iters, b->d_resid));
+    B_TRY(launch_batch_adr(b->stream, grid, block, b->group, b->view(), du, tol, max_iters, b->queue, b->d_iters,
+                           b->d_resid));
     B_TRY(hipEventRecord(e1, b->stream));
     B_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -237,6 +239,19 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     if (ms_out) *ms_out = ms;
     if (iters_out) B_TRY(hipMemcpy(iters_out, b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
     if (resid_out) B_TRY(hipMemcpy(resid_out, b->d_resid, sizeof(double) * nspec, hipMemcpyDeviceToHost));
+    return PD_OK;
+}
+
+int pd_batch_set_group(pd_batch *b, int group)
+{
+    // Lanes per row in the force phase: 1 (thread per particle), 4, 8, 16 or
+    // 32.  Every choice adds a row's bond terms in bond order.
+    if (!b) return PD_E_INVALID;
+    if (group != 1 && group != 4 && group != 8 && group != 16 && group != 32) {
+        b->err = "group must be 1, 4, 8, 16 or 32";
+        return PD_E_INVALID;
+    }
+    b->group = group;
     return PD_OK;
 }
 

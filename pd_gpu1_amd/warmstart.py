@@ -107,7 +107,7 @@ def node_features(pos, du_prev, dmg, top, bot):
 
 
 # ------------------------------------------------------------ training data
-def generate(specs, n_inc, tol=1e-6, max_iters=400000, block=1024, log=None):
+def generate(specs, n_inc, tol=1e-6, max_iters=400000, block=1024, group=4, log=None):
     """Run the batched ADR on ``specs`` for ``n_inc`` increments and keep every
     converged state.  Returns one record per specimen with its graph and the
     states u_0 = 0, u_1 .. u_n_inc (float64 [2][N]) and intact masks."""
@@ -115,6 +115,7 @@ def generate(specs, n_inc, tol=1e-6, max_iters=400000, block=1024, log=None):
     recs = []
     t0 = time.perf_counter()
     with PB.Batch(p, specs) as b:
+        b.set_group(group)
         for s, (n, _, _, _) in enumerate(specs):
             rp, col, mask = b.csr(s)
             x, y, n_int, top, bot = plate_layout(n)

@@ -42,14 +42,15 @@ def oracle_increment(m, du, tol, max_iters, T):
         it += 1
 
 
-@pytest.mark.parametrize("block", [256, 1024])
-def test_batch_adr_bit_exact_against_oracle(block):
+@pytest.mark.parametrize("block,group", [(256, 1), (1024, 1), (256, 8), (1024, 32)])
+def test_batch_adr_bit_exact_against_oracle(block, group):
     p = PB.plate_params()
     specs = [(12, 12, 0.0025, 0.3), (17, 17, 0.003, -0.2), (9, 9, 0.0, 0.0), (21, 21, 0.004, 0.5),
              (14, 14, 0.002, 0.0)]
     ms = [oracle_specimen(p, n, cl, a) for n, _, cl, a in specs]
     du, tol, max_it = 3.0e-6, 1e-6, 600
     with PB.Batch(p, specs) as b:
+        b.set_group(group)
         for s, m in enumerate(ms):
             rp, col, mask = b.csr(s)
             assert np.array_equal(rp, m.row_ptr) and np.array_equal(col, m.col) and np.array_equal(mask, m.fail)
