@@ -274,6 +274,10 @@ int pd_set_tile(pd_handle *h, int tile_cells);
  * indices.  mode -1 = where every tile fits (default), 0 = never (per-bond
  * gathers from L2), 1 = required (PD_E_CAPACITY otherwise).  Same results. */
 int pd_set_lds(pd_handle *h, int mode);
+/* Lanes per row of the FP64 PMB kernel: 1 (thread per row), 4, 8, 16, 32, or
+ * -1 = 4 in 2D and 16 in 3D (default).  Terms are added in bond order, so the
+ * forces and breaks are identical for every choice. */
+int pd_set_fp64_group(pd_handle *h, int group);
 /* Bond coefficients of the mixed kernels (|xi| and kappa or omega = w fac vol):
  * -1 = recomputed from the bond's reference length when every stored value
  * equals that recomputation to FP32 accuracy, which is checked once per

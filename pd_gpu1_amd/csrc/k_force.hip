@@ -66,6 +66,72 @@ __global__ __launch_bounds__(256) void k_pmb_fp64(const int *__restrict__ stop, 
     if (DIM == 3) f.p[2][i] = fz;
 }
 
+// The same force with G lanes per row: the lanes load G consecutive bonds
+// (coalesced col / kappa / idist), compute their terms with the arithmetic
+// above, and the group adds the terms in bond order through shuffles, so the
+// sums are the sequential loop's bit for bit.
+template <int DIM, int G>
+__global__ __launch_bounds__(256) void k_pmb_fp64_group(const int *__restrict__ stop, int n_rows,
+                                                        const int32_t *__restrict__ rp,
+                                                        const int32_t *__restrict__ col,
+                                                        const double *__restrict__ kappa,
+                                                        const double *__restrict__ idist, uint32_t *fail, CD3 x,
+                                                        CD3 u, D3 f, double s0, double ymin)
+{
+    if (stop && *stop) return;
+    const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / G);
+    const int lane = threadIdx.x & (G - 1);
+    if (i >= n_rows) return;  // the whole group leaves together
+    const double x0 = x.p[0][i], x1 = x.p[1][i], x2 = DIM == 3 ? x.p[2][i] : 0.0;
+    const double u0 = u.p[0][i], u1 = u.p[1][i], u2 = DIM == 3 ? u.p[2][i] : 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int32_t kb = rp[i], ke = rp[i + 1];
+    for (int32_t k0 = kb; k0 < ke; k0 += G) {
+        const int32_t k = k0 + lane;
+        double tx = 0.0, ty = 0.0, tz = 0.0;
+        int ok = 0;
+        if (k < ke && bond_intact(fail, k)) {
+            ok = 1;
+            const int c = col[k];
+            double d0 = x.p[0][c] + u.p[0][c] - x0 - u0;
+            double d1 = x.p[1][c] + u.p[1][c] - x1 - u1;
+            double r2 = d0 * d0 + d1 * d1;
+            double d2 = 0.0;
+            if (DIM == 3) {
+                d2 = x.p[2][c] + u.p[2][c] - x2 - u2;
+                r2 = r2 + d2 * d2;
+            }
+            double nl = sqrt(r2);
+            double id = idist[k];
+            double s = (nl - id) / id;
+            double ks = kappa[k] * s;
+            tx = ks * d0 / nl;
+            ty = ks * d1 / nl;
+            if (DIM == 3) {
+                tz = ks * d2 / nl;
+                if (s > s0 && x1 >= ymin && x.p[1][c] >= ymin) bond_break(fail, k);
+            } else {
+                if (s > s0) bond_break(fail, k);
+            }
+        }
+        const int cnt = ke - k0 < G ? ke - k0 : G;
+        for (int l = 0; l < cnt; ++l) {
+            const double sx = __shfl(tx, l, G), sy = __shfl(ty, l, G);
+            const double sz = DIM == 3 ? __shfl(tz, l, G) : 0.0;
+            if (__shfl(ok, l, G)) {
+                fx += sx;
+                fy += sy;
+                if (DIM == 3) fz += sz;
+            }
+        }
+    }
+    if (lane == 0) {
+        f.p[0][i] = fx;
+        f.p[1][i] = fy;
+        if (DIM == 3) f.p[2][i] = fz;
+    }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(256) void k_lps_theta_fp64(const int *__restrict__ stop, int N, const int32_t *__restrict__ rp,
                                                         const int32_t *__restrict__ col, CD3 x, CD3 u,
@@ -163,11 +229,36 @@ inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
 } // namespace
 
+namespace {
+template <int DIM, int G>
+void pmb_fp64_g(hipStream_t s, const int *stop, int n_rows, const int32_t *rp, const int32_t *col,
+                const double *kappa, const double *idist, uint32_t *fail, CD3 x, CD3 u, D3 f, double s0, double ymin)
+{
+    const int64_t threads = (int64_t)n_rows * G;
+    k_pmb_fp64_group<DIM, G><<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(stop, n_rows, rp, col, kappa, idist,
+                                                                                fail, x, u, f, s0, ymin);
+}
+} // namespace
+
 hipError_t launch_pmb_fp64(hipStream_t s, const int *stop, int dim, int n_rows, const int32_t *rp, const int32_t *col,
                            const double *kappa, const double *idist, uint32_t *fail, CD3 x, CD3 u,
-                           D3 f, double s0, double ymin)
+                           D3 f, double s0, double ymin, int group)
 {
     if (n_rows <= 0) return hipSuccess;
+    if (group == 4 || group == 8 || group == 16 || group == 32) {
+        if (dim == 3) {
+            if (group == 4) pmb_fp64_g<3, 4>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+            if (group == 8) pmb_fp64_g<3, 8>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+            if (group == 16) pmb_fp64_g<3, 16>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+            if (group == 32) pmb_fp64_g<3, 32>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+        } else {
+            if (group == 4) pmb_fp64_g<2, 4>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+            if (group == 8) pmb_fp64_g<2, 8>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+            if (group == 16) pmb_fp64_g<2, 16>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+            if (group == 32) pmb_fp64_g<2, 32>(s, stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
+        }
+        return hipGetLastError();
+    }
     if (dim == 3)
         k_pmb_fp64<3><<<nblk(n_rows, 256), 256, 0, s>>>(stop, n_rows, rp, col, kappa, idist, fail, x, u, f, s0, ymin);
     else

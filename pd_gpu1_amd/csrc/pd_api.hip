@@ -281,6 +281,15 @@ void lps_mixed_consts(const pd_params &P, float *ctheta, float *A, float *B)
     }
 }
 
+// Lanes per row of the FP64 PMB kernel: set explicitly, or 4 lanes for the
+// short 2D rows and 16 for 3D rows; the bond-order sum makes every choice
+// give the same bits.
+int fp64_group(const pd_handle *h)
+{
+    if (h->fp64_group > 0) return h->fp64_group;
+    return h->P.dim == 3 ? 16 : 4;
+}
+
 // Mixed mode: bring the u half of the particle records up to date when some
 // call changed u without refreshing them (the integrators and the contact
 // kernel refresh it themselves).  Runs before a graph is captured or
@@ -308,7 +317,7 @@ int force_launch(pd_handle *h, const int *stop)
         // all rows i < N (Force_Bond.cu:169)
         if (P.precision == PD_PREC_FP64) {
             LAUNCH(launch_pmb_fp64(s, stop, P.dim, h->N, h->row_ptr, h->col, h->kappa, h->idist, h->fail,
-                                   cd3(h->x), cd3(h->u), h->f, P.s0, ymin));
+                                   cd3(h->x), cd3(h->u), h->f, P.s0, ymin, fp64_group(h)));
         } else {
             LAUNCH(sync_rec_u(h));
             if (h->lds_ok)
@@ -1014,6 +1023,18 @@ int pd_set_lds(pd_handle *h, int mode)
     REQUIRE(mode >= -1 && mode <= 1, "mode must be -1, 0 or 1");
     h->lds_mode = mode;
     h->coef_dirty = true;
+    drop_graph(h);
+    return PD_OK;
+}
+
+int pd_set_fp64_group(pd_handle *h, int group)
+{
+    // Lanes per row of the FP64 PMB kernel: 1 (thread per row), 4, 8, 16,
+    // 32, or -1 for the choice by dimension.  Results are identical.
+    if (!h) return PD_E_INVALID;
+    REQUIRE(group == -1 || group == 1 || group == 4 || group == 8 || group == 16 || group == 32,
+            "group must be -1, 1, 4, 8, 16 or 32");
+    h->fp64_group = group;
     drop_graph(h);
     return PD_OK;
 }

@@ -74,6 +74,7 @@ struct pd_handle {
     bool lds_ok = false;          // the LDS tile lists are built and in use
     int lds_block = 512;          // threads per workgroup of the LDS kernels (pass 0)
     int lds_block2 = 512;         // ... of the LPS force pass
+    int fp64_group = -1;          // lanes per row of the FP64 PMB kernel (1 = thread per row, -1 = by row length)
     LdsTiles lds;
     double *react[3] = {nullptr, nullptr, nullptr};
 
@@ -173,7 +174,7 @@ hipError_t launch_damage(hipStream_t s, int N_int, const int32_t *rp, const uint
 // k_force.hip
 hipError_t launch_pmb_fp64(hipStream_t s, const int *stop, int dim, int n_rows, const int32_t *rp, const int32_t *col,
                            const double *kappa, const double *idist, uint32_t *fail, CD3 x, CD3 u,
-                           D3 f, double s0, double ymin);
+                           D3 f, double s0, double ymin, int group);
 hipError_t launch_lps_theta_fp64(hipStream_t s, const int *stop, int dim, int N, const int32_t *rp, const int32_t *col,
                                  CD3 x, CD3 u, const double *idist, const double *fac, const double *w,
                                  const double *m, const uint32_t *fail, double vol, double pratio,

@@ -175,6 +175,10 @@ class Problem:
         """LDS-staged tiles: -1 where they fit (default), 0 off, 1 required."""
         self._c(lib().pd_set_lds(self._h, mode))
 
+    def set_fp64_group(self, group):
+        """Lanes per row of the FP64 PMB kernel (1, 4, 8, 16, 32; -1 = by dimension)."""
+        self._c(lib().pd_set_fp64_group(self._h, group))
+
     def set_coef_mode(self, mode):
         """Bond coefficients: -1 recompute from |xi| where they match (default), 0 stream, 1 recompute."""
         self._c(lib().pd_set_coef_mode(self._h, mode))

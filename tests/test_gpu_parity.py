@@ -445,3 +445,26 @@ def test_surface_corrected_coefficients_are_streamed():
         prob.set_coef_mode(1)
         with pytest.raises(pd.PdError):
             prob.force()
+
+
+@pytest.mark.parametrize("case", ["kw3d", "c0_2d"])
+def test_fp64_pmb_lane_groups_are_bit_identical(case):
+    # G lanes per row add the bond terms in bond order: same bits as one thread per row
+    if case == "kw3d":
+        sc = scenarios.c1_kalthoff_winkler(steps=10)
+        rng = np.random.default_rng(51)
+        u = np.stack([4e-3 * sc.x, 4e-3 * sc.y, 1e-3 * sc.z]) + rng.uniform(-1e-7, 1e-7, (3, len(sc.x)))
+    else:
+        sc = scenarios.c0_plate_pmb(scale=0.6)
+        rng = np.random.default_rng(52)
+        u = np.stack([4.1e-3 * sc.x, 4e-3 * sc.y]) + rng.uniform(-1e-8, 1e-8, (2, len(sc.x)))
+    outs = []
+    for g in (1, 4, 16, 32):
+        with sc.make(precision=_lib.PD_PREC_FP64) as prob:
+            prob.set_fp64_group(g)
+            prob.set("U", u)
+            prob.force()
+            outs.append((prob.get("F"), prob.fail_mask()))
+    assert (outs[0][1] == 0).sum() > 0
+    for f, mk in outs[1:]:
+        assert np.array_equal(f, outs[0][0]) and np.array_equal(mk, outs[0][1])
