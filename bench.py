@@ -183,6 +183,54 @@ def pmb_section(steps=50, warmup=5, reps=10):
                          "ms_per_launch": ms_force, "traffic": load_traffic(kname.split("<")[0])}}
 
 
+def small_configs(cpu=True):
+    """The 2D PMB plate (C0) and the Kalthoff-Winkler impact (C1) in full:
+    GPU wall time of the whole run (FP64, and mixed for C1) and, with cpu,
+    the FP64 oracle on a bounded number of steps of the same run."""
+    from pd_gpu1_amd import _lib, scenarios
+
+    out = {}
+    for name, sc, precs, cpu_steps in (
+            ("c0_plate_pmb", scenarios.c0_plate_pmb(), (_lib.PD_PREC_FP64,), 200),
+            ("c1_kalthoff_winkler", scenarios.c1_kalthoff_winkler(), (_lib.PD_PREC_FP64, _lib.PD_PREC_MIXED), 10)):
+        row = {"steps": sc.steps}
+        for prec in precs:
+            with sc.make(precision=prec) as prob:
+                nb = prob.num_bonds
+                prob.sync()
+                t0 = time.perf_counter()
+                prob.run_verlet(sc.steps)
+                prob.sync()
+                wall = time.perf_counter() - t0
+                prob.damage()
+                broken = int((prob.fail_mask() == 0).sum())
+            key = "fp64" if prec == _lib.PD_PREC_FP64 else "mixed"
+            row[key] = {"wall_s": wall, "ms_per_step": wall * 1e3 / sc.steps,
+                        "bond_updates_per_s": nb * sc.steps / wall, "broken_bonds": broken}
+        row["bonds"] = nb
+        if cpu:
+            from oracle.runner import OracleModel
+            m = OracleModel(scenarios.params_dict(sc.p), sc.x, sc.y, sc.z, sc.n_int)
+            m.build_neighbors()
+            if sc.cyl is not None:
+                m.surface_correct()
+            m.init_state()
+            if sc.cyl is not None:
+                m.set_notches_kw(0.025)
+                m.cylinder_init(sc.cyl)
+            else:
+                m.set_crack_2d(0.01, 0.0, 0.0, 0.0)
+            t0 = time.perf_counter()
+            for _ in range(cpu_steps):
+                m.step()
+            dt = time.perf_counter() - t0
+            row["cpu_fp64_oracle"] = {"bond_updates_per_s": len(m.col) * cpu_steps / dt,
+                                      "sample": f"first {cpu_steps} steps", "seconds": dt,
+                                      "threads": int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()}
+        out[name] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +240,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adr", action="store_true", help="skip the ADR / warm-start measurements")
     ap.add_argument("--no-pmb", action="store_true", help="skip the PMB block measurement")
+    ap.add_argument("--no-small", action="store_true", help="skip the C0 plate and Kalthoff-Winkler runs")
     ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
     ap.add_argument("--shape", default="", metavar="TILE,B0,B1,G0,G1",
                     help="fixed LDS launch shape instead of autotuning: tile edge (dx), workgroup size and "
@@ -301,6 +350,10 @@ def main():
         t = time.perf_counter()
         out["pmb_block"] = pmb_section()
         print(f"[bench] PMB block {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    if not args.no_small:
+        t = time.perf_counter()
+        out["other_configs"] = small_configs(cpu=not args.no_cpu_baseline)
+        print(f"[bench] C0 and Kalthoff-Winkler runs {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_adr:
         t = time.perf_counter()
         out["adr"] = adr_section()
