@@ -104,7 +104,7 @@ def cpu_baseline(sample_dims=(200, 200, 100), evals=40):
     }
 
 
-def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6):
+def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log=False, max_iters=12000):
     """Batched ADR timing and the warm-start comparison (see module doc)."""
     import numpy as np
     from pd_gpu1_amd import batch as PB
@@ -116,17 +116,34 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6):
     out = {}
     specs = PB.plate_family(n_batch)
     block, group, tuned = autotune.tune_batch(PB.plate_params(), specs[:32])
-    with PB.Batch(PB.plate_params(), specs) as b:
-        b.set_group(group)
-        n_part, n_bonds, _ = b.info()
-        incs = []
-        for _ in range(batch_increments):
-            its, res, ms = b.increment(W.DU, 1e-6, 400000, block=block)
-            incs.append({"wall_ms": round(ms, 1), "iterations_median": int(np.median(its)),
-                         "iterations_max": int(its.max()), "max_residual": float(res.max())})
-    out["batched"] = {"workload": f"{n_batch} PMB plates, sides 40..200 (seed 0x5EED0003+k), du=1e-6 m, "
-                                  "ADR to relative residual 1e-6, one persistent launch per increment",
-                      "particles": n_part, "bonds": n_bonds, "increments": incs,
+    runs = {}
+    for mode in (0, 1):
+        with PB.Batch(PB.plate_params(), specs) as b:
+            b.set_group(group)
+            b.set_warm(mode)
+            b.set_retire(1)
+            n_part, n_bonds, _ = b.info()
+            incs = []
+            for k in range(batch_increments):
+                its, res, ms = b.increment(W.DU, 1e-6, max_iters, block=block)
+                active = its > 0
+                unconv = int(((res > 1e-6) & active).sum())
+                incs.append({"wall_ms": round(ms, 1), "active": int(active.sum()),
+                             "iterations_median": int(np.median(its[active])) if active.any() else 0,
+                             "iterations_max": int(its.max()), "unconverged": unconv})
+                if log:
+                    print(f"[bench] batch {'warm' if mode else 'plain'} increment {k}: {ms:.0f} ms, "
+                          f"{int(active.sum())} active, iterations median {incs[-1]['iterations_median']} "
+                          f"max {its.max()}, unconverged {unconv}", file=sys.stderr, flush=True)
+            runs["warm" if mode else "plain"] = incs
+    tot = {k: sum(i["wall_ms"] for i in v) for k, v in runs.items()}
+    out["batched"] = {"workload": f"{n_batch} PMB plates, sides 40..200 (seed 0x5EED0003+k), {batch_increments} "
+                                  "increments of du=1e-6 m, ADR to relative residual 1e-6, one persistent launch "
+                                  f"per increment; a specimen that has not converged after {max_iters} iterations "
+                                  "(fractured through) is retired",
+                      "particles": n_part, "bonds": n_bonds, "plain_s": tot["plain"] / 1e3,
+                      "warm_s": tot["warm"] / 1e3, "speedup": tot["plain"] / tot["warm"],
+                      "increments_plain": runs["plain"], "increments_warm": runs["warm"],
                       "launch_shape": {"block": block, "lanes_per_row": group}, "autotune_ms_per_iteration": tuned}
     wpath = os.path.join(ROOT, "pd_gpu1_amd", "data", "mpnn_warmstart.npy")
     specs = W.eval_specs(n_eval)
@@ -241,6 +258,8 @@ def main():
     ap.add_argument("--no-adr", action="store_true", help="skip the ADR / warm-start measurements")
     ap.add_argument("--no-pmb", action="store_true", help="skip the PMB block measurement")
     ap.add_argument("--no-small", action="store_true", help="skip the C0 plate and Kalthoff-Winkler runs")
+    ap.add_argument("--c3-full", action="store_true",
+                    help="batched ADR at the full family size (512 plates x 50 increments, several minutes)")
     ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
     ap.add_argument("--shape", default="", metavar="TILE,B0,B1,G0,G1",
                     help="fixed LDS launch shape instead of autotuning: tile edge (dx), workgroup size and "
@@ -356,7 +375,7 @@ def main():
         print(f"[bench] C0 and Kalthoff-Winkler runs {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_adr:
         t = time.perf_counter()
-        out["adr"] = adr_section()
+        out["adr"] = adr_section(n_batch=512, batch_increments=50, log=True) if args.c3_full else adr_section()
         print(f"[bench] ADR measurements {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

@@ -225,6 +225,15 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
  * 4, 8, 16, 32 lanes per particle; every choice adds a row's bond terms in
  * bond order, so results are identical. */
 int pd_batch_set_group(pd_batch *b, int group);
+/* Warm start of the batched increments (no reference counterpart): 0 = plain
+ * ADR from the previous solution with the strips moved (default), 1 = the
+ * free particles also move by the previous increment, which is exact while
+ * the response is linear; ADR still converges every specimen to tol. */
+int pd_batch_set_warm(pd_batch *b, int mode);
+/* on = 1: a specimen whose increment ends without reaching tol (it has
+ * separated, or needs more than max_iters) is retired; later increments skip
+ * it and report 0 iterations (no reference counterpart). */
+int pd_batch_set_retire(pd_batch *b, int on);
 int pd_batch_get_field(pd_batch *b, int spec, int field, double *out);       /* [2][n] */
 int pd_batch_set_field(pd_batch *b, int spec, int field, const double *in);
 int pd_batch_get_csr(pd_batch *b, int spec, int32_t *row_ptr, int32_t *col, uint32_t *fail_words);

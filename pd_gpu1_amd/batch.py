@@ -98,6 +98,14 @@ class Batch:
         """Lanes per particle of the force phase (1 = thread per particle)."""
         self._c(lib().pd_batch_set_group(self._h, group))
 
+    def set_warm(self, mode):
+        """1: start each increment from u + the previous increment (0: plain)."""
+        self._c(lib().pd_batch_set_warm(self._h, mode))
+
+    def set_retire(self, on):
+        """1: skip specimens from the increment after one that ended unconverged."""
+        self._c(lib().pd_batch_set_retire(self._h, on))
+
     def increment(self, du, tol, max_iters, block=1024, grid=0):
         its = np.zeros(self.nspec, dtype=np.int32)
         res = np.zeros(self.nspec)

@@ -73,6 +73,10 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         const int s = spec_sh;
         __syncthreads();
         if (s >= B.nspec) break;
+        if (B.retired && B.retired[s]) {
+            if (tid == 0) iters_out[s] = 0;
+            continue;  // uniform: every thread read the same s
+        }
         const BatchSpec d = B.spec[s];
         const int N = d.n, Ni = d.n_int;
         const double *X = B.x + d.p_off, *Y = B.y + d.p_off;
@@ -87,6 +91,17 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         const double *kap = B.kappa + d.b_off, *idist = B.idist + d.b_off;
         uint32_t *fail = B.fail + (d.b_off >> 5);
         const double mass = B.mass, dt = B.dt, s0 = B.s0;
+        double *DUPX = B.dupx + d.p_off, *DUPY = B.dupy + d.p_off;
+        if (B.warm) {
+            // start from u + the previous increment; keep the start state
+            for (int i = tid; i < Ni; i += BT) {
+                const double sx = UX[i], sy = UY[i];
+                UX[i] = sx + DUPX[i];
+                UY[i] = sy + DUPY[i];
+                DUPX[i] = sx;
+                DUPY[i] = sy;
+            }
+        }
         // displacement control of the strips (Solve_Plate_Crack_Grow.cu:20-27 layout)
         for (int k = tid; k < d.n_top + d.n_bot; k += BT) UY[Ni + k] += k < d.n_top ? du : -du;
         __syncthreads();
@@ -204,9 +219,16 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
             __syncthreads();
             ++it;
         }
+        if (B.warm) {
+            for (int i = tid; i < Ni; i += BT) {
+                DUPX[i] = UX[i] - DUPX[i];
+                DUPY[i] = UY[i] - DUPY[i];
+            }
+        }
         if (tid == 0) {
             iters_out[s] = it;
             resid_out[s] = r;
+            if (B.retired && !(r <= tol)) B.retired[s] = 1;
         }
         __syncthreads();
     }

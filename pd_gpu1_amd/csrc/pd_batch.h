@@ -15,6 +15,9 @@ struct BatchView {
     const BatchSpec *spec;
     const double *x, *y;
     double *ux, *uy, *fx, *fy, *vx, *vy, *vhx, *vhy, *vhox, *vhoy, *fox, *foy;
+    double *dupx, *dupy;  // previous increment of u (warm start), then the start state
+    int warm;             // 1: start each increment from u + previous increment
+    int32_t *retired;     // non-null: skip specimens marked 1, mark those that end unconverged
     const int32_t *row_ptr, *col;
     const double *kappa, *idist;
     uint32_t *fail;

@@ -19,25 +19,31 @@ struct pd_batch {
     std::vector<BatchSpec> spec;
     BatchSpec *d_spec = nullptr;
     int64_t n_total = 0, nb_total = 0, rp_total = 0;
-    double *buf = nullptr;        // 14 particle fields of n_total doubles
+    double *buf = nullptr;        // 16 particle fields of n_total doubles
     int32_t *row_ptr = nullptr, *col = nullptr;
     double *kappa = nullptr, *idist = nullptr;
     uint32_t *fail = nullptr;
     unsigned *queue = nullptr;
     int32_t *d_iters = nullptr;
     double *d_resid = nullptr;
+    int32_t *d_retired = nullptr; // per specimen: 1 once an increment ended unconverged (with retire on)
+    int retire = 0;
     int last_block = 1024;
     int group = 1;                // lanes per row of the force phase (1 = thread per particle)
+    int warm = 0;                 // 1: increments start from u + the previous increment
     BatchView view() const
     {
         BatchView v{};
         v.nspec = (int)spec.size();
         v.spec = d_spec;
-        double *f[14];
-        for (int k = 0; k < 14; k++) f[k] = buf + (size_t)k * n_total;
+        double *f[16];
+        for (int k = 0; k < 16; k++) f[k] = buf + (size_t)k * n_total;
         v.x = f[0]; v.y = f[1]; v.ux = f[2]; v.uy = f[3]; v.fx = f[4]; v.fy = f[5];
         v.vx = f[6]; v.vy = f[7]; v.vhx = f[8]; v.vhy = f[9]; v.vhox = f[10]; v.vhoy = f[11];
         v.fox = f[12]; v.foy = f[13];
+        v.dupx = f[14]; v.dupy = f[15];
+        v.warm = warm;
+        v.retired = retire ? d_retired : nullptr;
         v.row_ptr = row_ptr; v.col = col; v.kappa = kappa; v.idist = idist; v.fail = fail;
         v.mass = P.mass; v.dt = P.dt; v.s0 = P.s0;
         return v;
@@ -56,7 +62,7 @@ struct pd_batch {
 static void batch_free(pd_batch *b)
 {
     void *ptrs[] = {b->d_spec, b->buf, b->row_ptr, b->col, b->kappa, b->idist, b->fail, b->queue, b->d_iters,
-                    b->d_resid};
+                    b->d_resid, b->d_retired};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -120,7 +126,7 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
     auto alloc = [&](void **p, size_t bytes) {
         return hipMalloc(p, bytes ? bytes : 16) == hipSuccess ? PD_OK : PD_E_HIP;
     };
-    if (!rc) rc = alloc((void **)&b->buf, sizeof(double) * 14 * b->n_total);
+    if (!rc) rc = alloc((void **)&b->buf, sizeof(double) * 16 * b->n_total);
     if (!rc) rc = alloc((void **)&b->row_ptr, sizeof(int32_t) * b->rp_total);
     if (!rc) rc = alloc((void **)&b->col, sizeof(int32_t) * b->nb_total);
     if (!rc) rc = alloc((void **)&b->kappa, sizeof(double) * b->nb_total);
@@ -130,7 +136,9 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
     if (!rc) rc = alloc((void **)&b->queue, sizeof(unsigned));
     if (!rc) rc = alloc((void **)&b->d_iters, sizeof(int32_t) * nspec);
     if (!rc) rc = alloc((void **)&b->d_resid, sizeof(double) * nspec);
-    if (!rc && hipMemset(b->buf, 0, sizeof(double) * 14 * b->n_total) != hipSuccess) rc = PD_E_HIP;
+    if (!rc) rc = alloc((void **)&b->d_retired, sizeof(int32_t) * nspec);
+    if (!rc && hipMemset(b->d_retired, 0, sizeof(int32_t) * nspec) != hipSuccess) rc = PD_E_HIP;
+    if (!rc && hipMemset(b->buf, 0, sizeof(double) * 16 * b->n_total) != hipSuccess) rc = PD_E_HIP;
     for (int s = 0; s < nspec && !rc; s++) {
         pd_handle *h = hs[s];
         const BatchSpec &d = b->spec[s];
@@ -239,6 +247,26 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     if (ms_out) *ms_out = ms;
     if (iters_out) B_TRY(hipMemcpy(iters_out, b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
     if (resid_out) B_TRY(hipMemcpy(resid_out, b->d_resid, sizeof(double) * nspec, hipMemcpyDeviceToHost));
+    return PD_OK;
+}
+
+int pd_batch_set_warm(pd_batch *b, int mode)
+{
+    // 0: each increment starts from the previous solution with the strips
+    // moved (plain ADR); 1: the free particles also move by the previous
+    // increment (exact for a linear response, the first increment is plain).
+    if (!b || (mode != 0 && mode != 1)) return PD_E_INVALID;
+    b->warm = mode;
+    return PD_OK;
+}
+
+int pd_batch_set_retire(pd_batch *b, int on)
+{
+    // 1: a specimen whose increment ends without reaching tol (it has
+    // separated, or needs more than max_iters) is retired: later increments
+    // skip it and report 0 iterations for it.
+    if (!b || (on != 0 && on != 1)) return PD_E_INVALID;
+    b->retire = on;
     return PD_OK;
 }
 

@@ -72,3 +72,41 @@ def test_batch_adr_converges_to_tolerance():
             its, res, ms = b.increment(1.0e-6, 1e-6, 200000, block=1024)
             print("increment", inc, "iterations", its.tolist(), "ms", round(ms, 2), flush=True)
             assert np.all(res <= 1e-6), res
+
+
+def test_batch_warm_start_reaches_the_same_states_in_fewer_iterations():
+    p = PB.plate_params()
+    specs = PB.plate_family(12, n_min=24, n_max=60)
+    its_p, its_w = [], []
+    with PB.Batch(p, specs) as plain, PB.Batch(p, specs) as warm:
+        warm.set_warm(1)
+        for inc in range(4):
+            a, ra, _ = plain.increment(1.0e-6, 1e-6, 200000, block=1024)
+            b, rb, _ = warm.increment(1.0e-6, 1e-6, 200000, block=1024)
+            assert np.all(ra <= 1e-6) and np.all(rb <= 1e-6)
+            its_p.append(a)
+            its_w.append(b)
+        for s in range(len(specs)):
+            up, uw = plain.get(s, "U"), warm.get(s, "U")
+            assert np.max(np.abs(up - uw)) < 1e-3 * np.max(np.abs(up)), s
+            assert np.array_equal(plain.csr(s)[2], warm.csr(s)[2])
+    assert np.array_equal(its_p[0], its_w[0])  # no history: the first increment is plain
+    assert sum(x.sum() for x in its_w[1:]) < 0.5 * sum(x.sum() for x in its_p[1:])
+
+
+def test_batch_retire_skips_only_unconverged_specimens():
+    p = PB.plate_params()
+    specs = PB.plate_family(6, n_min=24, n_max=40)
+    with PB.Batch(p, specs) as cut, PB.Batch(p, specs) as ok:
+        cut.set_retire(1)
+        ok.set_retire(1)
+        a, ra, _ = cut.increment(1.0e-6, 1e-6, 20, block=1024)  # 20 iterations reach no specimen's tol
+        assert np.all(a == 20) and np.all(ra > 1e-6)
+        before = [cut.get(s, "U").copy() for s in range(len(specs))]
+        a2, _, _ = cut.increment(1.0e-6, 1e-6, 20, block=1024)
+        assert np.all(a2 == 0)
+        for s in range(len(specs)):
+            assert np.array_equal(cut.get(s, "U"), before[s]), s
+        for inc in range(2):  # converged increments never retire a specimen
+            b, rb, _ = ok.increment(1.0e-6, 1e-6, 200000, block=1024)
+            assert np.all(rb <= 1e-6) and np.all(b > 0)

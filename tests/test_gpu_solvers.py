@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import pd_gpu1_amd as pd
-from pd_gpu1_amd import _lib
+from pd_gpu1_amd import _lib, scenarios
 
 pytestmark = pytest.mark.gpu
 
@@ -73,3 +73,15 @@ def test_kalthoff_winkler_solver_layout(tmp_path):
     vals = np.array([[float(t) for t in s.split()] for s in lines if not s.startswith("nan")])
     assert np.all(np.isfinite(vals)) and vals.shape[1] == 4
     assert vals[:, 1].min() < 0  # the impactor pushed the top edge down
+
+
+def test_crack_tip_advances_in_the_pulled_plate():
+    from pd_gpu1_amd import observables as ob
+    sc = scenarios.c0_plate_pmb()
+    nx, ny, _ = sc.nxyz
+    with sc.make() as prob:
+        times, lengths = ob.track_crack(prob, nx, ny, sc.p.dx, sc.p.dt, 1000, every=100)
+    print("crack half-length", lengths)
+    assert lengths[0] >= 0.005 - 2 * sc.p.dx       # the 10 mm pre-crack
+    assert lengths[-1] > lengths[0]                 # the crack has grown
+    assert ob.crack_speed(times, lengths) > 0.0

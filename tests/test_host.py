@@ -20,3 +20,16 @@ def test_splitmix64_reference_values():
     assert v == 0xE220A8397B1DCDAF
     v, st = PB.splitmix64(st)
     assert v == 0x6E789E6AA1B965F4
+
+
+def test_crack_half_length_and_speed():
+    from pd_gpu1_amd import observables as ob
+    nx, ny, dx = 20, 9, 1e-3
+    dmg = np.zeros(nx * ny)
+    c = (ny // 2) * nx
+    dmg[c + 6: c + 14] = 0.5          # damaged band centred on the plate
+    dmg[c + 5] = 0.30                 # below the threshold
+    assert ob.crack_half_length(dmg, nx, ny, dx) == (nx // 2 - 6) * dx
+    assert ob.crack_half_length(np.zeros(nx * ny), nx, ny, dx) == 0.0
+    t = np.array([0.0, 1e-6, 2e-6, 3e-6])
+    assert abs(ob.crack_speed(t, 0.004 + 500.0 * t) - 500.0) < 1e-6
