@@ -259,7 +259,7 @@ def main():
     ap.add_argument("--no-pmb", action="store_true", help="skip the PMB block measurement")
     ap.add_argument("--no-small", action="store_true", help="skip the C0 plate and Kalthoff-Winkler runs")
     ap.add_argument("--c3-full", action="store_true",
-                    help="batched ADR at the full family size (512 plates x 50 increments, several minutes)")
+                    help="batched ADR at the full family size (512 plates x 50 increments, plain and warm: about 17 minutes)")
     ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
     ap.add_argument("--shape", default="", metavar="TILE,B0,B1,G0,G1",
                     help="fixed LDS launch shape instead of autotuning: tile edge (dx), workgroup size and "
@@ -375,7 +375,8 @@ def main():
         print(f"[bench] C0 and Kalthoff-Winkler runs {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_adr:
         t = time.perf_counter()
-        out["adr"] = adr_section(n_batch=512, batch_increments=50, log=True) if args.c3_full else adr_section()
+        out["adr"] = (adr_section(n_batch=512, batch_increments=50, log=True, max_iters=6000) if args.c3_full
+                      else adr_section())
         print(f"[bench] ADR measurements {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

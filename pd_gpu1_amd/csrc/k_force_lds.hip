@@ -7,8 +7,12 @@
 // are copied into LDS once per pass (four consecutive lanes move one 64-byte
 // record, so the copy is a coalesced stream of whole records), and every bond
 // of the tile then reads its neighbour from LDS through a 16-bit local index
-// lidx[k] instead of gathering 64 bytes from the L2 per bond.  The bond
-// stream shrinks to lidx (2 B) + {|xi|, coefficient} (8 B) + the fail bit.
+// lidx[k] instead of gathering 64 bytes from the L2 per bond.  Bit 15 of
+// lidx[k] mirrors the bond's fail bit (1 = intact), so one 2-byte load per
+// bond serves both; the fail words stay the bonds' state, breaks clear both,
+// and k_lidx_sync copies the fail words into bit 15 after any other change.
+// The bond stream shrinks to lidx (2 B) + {|xi|, coefficient} (8 B, none
+// when the coefficients are recomputed).
 //
 // Arithmetic, bond slots, the group reduction and the break decision are the
 // ones of the gathering kernels (mixed_bond.h) with the same bond-to-lane
@@ -24,7 +28,8 @@ constexpr int kHashSlots = 8192;  // per-tile open-addressing table of the build
 // start on different bank groups (a 64-byte stride puts every record on the
 // same 4 of the 64 banks and serialises the 16-byte reads of a wave).
 constexpr int kLdsRec2 = 5;
-constexpr int kStageCap = 4096;   // largest staged set the builder handles
+constexpr int kStageCap = 4096;   // largest staged set the builder handles (< 2^15)
+constexpr uint32_t kIntact = 0x8000u;  // bit 15 of a local index word
 
 // lowbias32 integer hash (C. Wellons, public domain)
 __device__ __forceinline__ uint32_t hash32(uint32_t v)
@@ -130,6 +135,15 @@ __global__ __launch_bounds__(1024) void k_stage_tiles(int phase, const int32_t *
     }
 }
 
+// Copy every bond's fail bit into bit 15 of its local index word.
+__global__ void k_lidx_sync(int64_t nb, const uint32_t *__restrict__ fail, uint16_t *lidx)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const uint32_t bit = (fail[k >> 5] >> (k & 31)) & 1u;
+    lidx[k] = (uint16_t)((lidx[k] & (kIntact - 1)) | (bit << 15));
+}
+
 // Tiles handed to the 8 XCDs in contiguous ranges (gridDim.x is a multiple of 8).
 __device__ __forceinline__ int xcd_tile() { return xcd_block(); }
 
@@ -152,18 +166,23 @@ __device__ __forceinline__ RecQ lds_rec(const double2 *lds, int l)
     return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
 }
 
+// 2-byte load at a 32-bit byte offset from a uniform base (no 64-bit
+// address arithmetic per bond; Nb < 2^31).
+__device__ __forceinline__ uint32_t load_lidx(const uint16_t *lidx, uint32_t k)
+{
+    return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(lidx) + 2u * k);
+}
+
 template <bool GC>
-__device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_t *__restrict__ lidx,
-                                            const float2 *__restrict__ bnd, const uint32_t *__restrict__ fail,
-                                            bool need_intact)
+__device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_t *lidx,
+                                            const float2 *__restrict__ bnd, bool need_intact)
 {
     const bool valid = k < ke;
-    const int32_t kk = valid ? k : 0;
-    const int l = lidx[kk];
+    const uint32_t kk = valid ? (uint32_t)k : 0u;
+    const uint32_t v = load_lidx(lidx, kk);
     const float2 b = GC ? make_float2(1.f, 0.f) : bnd[kk];
-    const uint32_t w = fail[kk >> 5];
-    const bool act = valid && (!need_intact || ((w >> (k & 31)) & 1u));
-    return Slot{valid ? l : 0, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
+    const bool act = valid && (!need_intact || (v & kIntact));
+    return Slot{valid ? (int)(v & (kIntact - 1)) : 0, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
 }
 
 // Row walk over LDS records; same lane/bond assignment as the gathering walk.
@@ -171,12 +190,12 @@ __device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_
     {                                                                                            \
         const int32_t kb = rp[i], ke = rp[i + 1];                                                \
         int32_t k = kb + lane;                                                                   \
-        Slot sa = fetch_lslot<GC>(k, ke, lidx, bnd, fail, NEED_INTACT);                              \
-        Slot sb = fetch_lslot<GC>(k + G, ke, lidx, bnd, fail, NEED_INTACT);                          \
+        Slot sa = fetch_lslot<GC>(k, ke, lidx, bnd, NEED_INTACT);                                        \
+        Slot sb = fetch_lslot<GC>(k + G, ke, lidx, bnd, NEED_INTACT);                                    \
         while (k < ke) {                                                                         \
             const RecQ ra = lds_rec(lds, sa.c), rb = lds_rec(lds, sb.c);                         \
-            const Slot na = fetch_lslot<GC>(k + 2 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
-            const Slot nb = fetch_lslot<GC>(k + 3 * G, ke, lidx, bnd, fail, NEED_INTACT);            \
+            const Slot na = fetch_lslot<GC>(k + 2 * G, ke, lidx, bnd, NEED_INTACT);                      \
+            const Slot nb = fetch_lslot<GC>(k + 3 * G, ke, lidx, bnd, NEED_INTACT);                      \
             BODY(ra, sa, k);                                                                     \
             BODY(rb, sb, k + G);                                                                 \
             sa = na;                                                                             \
@@ -200,7 +219,7 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
                                                   const int32_t *__restrict__ order,
                                                   const int64_t *__restrict__ stage_ptr,
                                                   const int32_t *__restrict__ stage_idx,
-                                                  const int32_t *__restrict__ rp, const uint16_t *__restrict__ lidx,
+                                                  const int32_t *__restrict__ rp, uint16_t *lidx,
                                                   const float2 *__restrict__ bnd, uint32_t *fail,
                                                   const double *__restrict__ rec, F3 f, double s0, float s0f,
                                                   float win, double ymin, GeomCoef gc)
@@ -215,13 +234,13 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
-        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
+        if (GC) geom_coef<false>(g.xi2, gc, L0, cf, rL0);                                        \
         const float s = g.num * frcp(L0 * (g.L + L0));                                           \
         const float tb = masked(cf * s * g.rL, S.mask);                                      \
         ax += (double)(tb * g.dx);                                                               \
         ay += (double)(tb * g.dy);                                                               \
         if (DIM == 3) az += (double)(tb * g.dz);                                                 \
-        check_break<DIM, false>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail);                     \
+        check_break<DIM, false>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail, lidx);               \
     }
         PD_LDS_ROW_WALK(true, PMB_BODY)
 #undef PMB_BODY
@@ -243,8 +262,7 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
                                                         const int32_t *__restrict__ stage_idx,
                                                         const int32_t *__restrict__ rp,
                                                         const uint16_t *__restrict__ lidx,
-                                                        const float2 *__restrict__ bnd,
-                                                        const uint32_t *__restrict__ fail, double *rec,
+                                                        const float2 *__restrict__ bnd, double *rec,
                                                         const float *__restrict__ minv32, float ctheta,
                                                         int skip_broken, float *theta32, GeomCoef gc)
 {
@@ -262,7 +280,7 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
-        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
+        if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
         const float e = g.num * frcp(g.L + L0); /* |xi+eta| - |xi| */                            \
         acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
@@ -284,8 +302,7 @@ __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ 
                                                         const int32_t *__restrict__ order,
                                                         const int64_t *__restrict__ stage_ptr,
                                                         const int32_t *__restrict__ stage_idx,
-                                                        const int32_t *__restrict__ rp,
-                                                        const uint16_t *__restrict__ lidx,
+                                                        const int32_t *__restrict__ rp, uint16_t *lidx,
                                                         const float2 *__restrict__ bnd, uint32_t *fail,
                                                         const double *__restrict__ rec, float A, float B,
                                                         double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc)
@@ -301,14 +318,14 @@ __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ 
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
-        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
+        if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
         const float e = g.num * frcp(g.L + L0);                                                  \
         const float s = e * (GC ? rL0 : frcp(L0));                                               \
         const float tt = masked(cf * (A * L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
-        check_break<DIM, true>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail);                      \
+        check_break<DIM, true>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail, lidx);                \
     }
         PD_LDS_ROW_WALK(true, LPSF_BODY)
 #undef LPSF_BODY
@@ -365,7 +382,7 @@ hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const in
 
 template <int DIM, int G, bool GC>
 hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
-                       const float2 *bnd, const uint32_t *fail, double *rec, const float *minv32, float ctheta,
+                       const float2 *bnd, double *rec, const float *minv32, float ctheta,
                        int skip_broken, float *theta32, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
@@ -373,7 +390,7 @@ hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const 
     if (e != hipSuccess) return e;
     k_lps_theta_lds<DIM, G, GC><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order,
                                                                           T.stage_ptr, T.stage_idx, rp, T.lidx, bnd,
-                                                                          fail, rec, minv32, ctheta, skip_broken,
+                                                                          rec, minv32, ctheta, skip_broken,
                                                                           theta32, gc);
     return hipGetLastError();
 }
@@ -403,6 +420,13 @@ hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_
     return hipGetLastError();
 }
 
+hipError_t launch_lidx_sync(hipStream_t s, int64_t nb, const uint32_t *fail, uint16_t *lidx)
+{
+    if (nb <= 0) return hipSuccess;
+    k_lidx_sync<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(nb, fail, lidx);
+    return hipGetLastError();
+}
+
 int stage_cap() { return kStageCap; }
 int lds_record_bytes() { return 16 * kLdsRec2; }
 
@@ -419,16 +443,16 @@ hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int
 }
 
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
-                                const int32_t *order, const int32_t *rp, const float2 *bnd, const uint32_t *fail,
+                                const int32_t *order, const int32_t *rp, const float2 *bnd,
                                 double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32,
                                 const GeomCoef *gc)
 {
     if (T.ntiles <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(theta_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta,
+        PD_DISPATCH_LDS(theta_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta,
                         skip_broken, theta32)
     } else {
-        PD_DISPATCH_LDS(theta_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, minv32, ctheta,
+        PD_DISPATCH_LDS(theta_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta,
                         skip_broken, theta32)
     }
 }

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
-        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
+        if (GC) geom_coef<false>(g.xi2, gc, L0, cf, rL0);                                        \
         const float s = g.num * frcp(L0 * (g.L + L0));                                           \
         const float t = masked(cf * s * g.rL, S.mask);                                       \
         ax += (double)(t * g.dx);                                                                \
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
-        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
+        if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
         const float e = g.num * frcp(g.L + L0); /* |xi+eta| - |xi| */                            \
         acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
         _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
-        if (GC) geom_coef(g.xi2, gc, L0, cf, rL0);                                               \
+        if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
         const float e = g.num * frcp(g.L + L0);                                                  \
         const float s = e * (GC ? rL0 : frcp(L0));                                               \
         const float tt = masked(cf * (A * L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
@@ -207,7 +207,10 @@ __global__ void k_geom_check(int dim, int N, const int32_t *__restrict__ rp, con
             xi2 += ez * ez;
         }
         float L0, cf, rL0;
-        geom_coef(xi2, gc, L0, cf, rL0);
+        if (gc.k2 != 0.f)
+            geom_coef<true>(xi2, gc, L0, cf, rL0);
+        else
+            geom_coef<false>(xi2, gc, L0, cf, rL0);
         const float2 b = bnd[k];
         if (!(fabsf(L0 - b.x) <= 1e-6f * b.x && fabsf(cf - b.y) <= 1e-5f * fabsf(b.y) + 1e-30f)) {
             atomicAnd(ok, 0);

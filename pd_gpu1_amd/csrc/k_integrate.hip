@@ -113,6 +113,33 @@ __global__ void k_verlet1(int dim, int N_int, int n_top, int n_bot, double dt, d
     }
 }
 
+// Half 2 of one step and half 1 of the next in one pass (runs without the
+// KW cylinder): the same operations in the same order as k_verlet2 followed
+// by k_verlet1, with the acceleration kept in a register.  The acceleration
+// of a step that another step follows is never read by the API, so only
+// the last step's k_verlet2 stores it.
+__global__ void k_verlet21(int dim, int N_int, int n_top, int n_bot, double dt, double dens, double load, D3 v,
+                           CD3 f, CF3 f32, int use32, CD3 b, D3 u, double *rec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N_int) {
+        for (int d = 0; d < dim; ++d) {
+            const double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
+            const double ad = (fd + b.p[d][i]) / dens;  // Integrate.cu:465-481
+            double vd = v.p[d][i] + (dt / 2.0) * ad;
+            vd = vd + (dt / 2.0) * ad;                   // Integrate.cu:447-463
+            v.p[d][i] = vd;
+            u.p[d][i] += vd * dt;
+        }
+    } else if (i < N_int + n_top) {
+        u.p[1][i] += load * dt;
+    } else if (i < N_int + n_top + n_bot) {
+        u.p[1][i] -= load * dt;
+    }
+    if (rec && i < N_int + n_top + n_bot)
+        store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
+}
+
 __global__ void k_verlet2(int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f, CF3 f32, int use32,
                           CD3 b)
 {
@@ -339,6 +366,15 @@ hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bo
     int n = N_int + n_top + n_bot;
     if (n < 1) n = 1;
     k_verlet1<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, load, v, a, u, cyl_state, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_verlet21(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double dens,
+                           double load, D3 v, CD3 f, CF3 f32, int use32, CD3 b, D3 u, double *rec)
+{
+    int n = N_int + n_top + n_bot;
+    if (n < 1) n = 1;
+    k_verlet21<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, dens, load, v, f, f32, use32, b, u, rec);
     return hipGetLastError();
 }
 

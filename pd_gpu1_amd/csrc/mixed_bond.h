@@ -98,7 +98,9 @@ __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
 // streamed from HBM (used when every stored coefficient equals this
 // recomputation to FP32 accuracy, which the engine checks once per set-up):
 // the partial-volume factor of Surf_Corr.cu:27-32, and for LPS the influence
-// weight exp(-xi^2/delta^2) of Force_State.cu:317.  coef = scale * fac [* w].
+// weight exp(-xi^2/delta^2) of Force_State.cu:317.  coef = scale * fac [* w],
+// the weight when W (the LPS passes; gc.k2 is then nonzero).
+template <bool W>
 __device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &L0, float &coef, float &rL0)
 {
 #pragma clang fp contract(fast)
@@ -107,7 +109,7 @@ __device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &
     // 1 up to lo = delta - dx/2, (hi - |xi|)/dx up to hi = lo + dx, 0 beyond
     const float fac = __builtin_amdgcn_fmed3f((gc.hi - L0) * gc.inv_dx, 0.f, 1.f);
     coef = gc.scale * fac;
-    if (gc.k2 != 0.f) coef *= __builtin_amdgcn_exp2f(gc.k2 * xi2);
+    if (W) coef *= __builtin_amdgcn_exp2f(gc.k2 * xi2);
 }
 
 // FP64 stretch with the parity expressions (PMB order (x_c+u_c)-x_i-u_i for
@@ -170,14 +172,21 @@ inline float stretch_window(double s0, double near_window)
     return (float)(w > 3.0e-6 ? w : 3.0e-6);
 }
 
+// Break decision of one bond.  lidx, when given, is the LDS tile kernels'
+// per-bond local index word whose bit 15 mirrors the fail bit; it is
+// cleared with the fail bit.
 template <int DIM, bool LPS3Z>
 __device__ __forceinline__ void check_break(const Rec &ri, const Rec &rc, float s, const Slot &sl, int32_t k,
-                                            float s0f, float win, double s0, double ymin, uint32_t *fail)
+                                            float s0f, float win, double s0, double ymin, uint32_t *fail,
+                                            uint16_t *lidx = nullptr)
 {
     if (sl.mask && s >= s0f - win) {
         const bool brk = (s > s0f + win ? true : stretch64<DIM, LPS3Z>(ri, rc) > s0) &&
                          (DIM == 2 || (ri.x1 >= ymin && rc.x1 >= ymin));
-        if (brk) bond_break(fail, k);
+        if (brk) {
+            bond_break(fail, k);
+            if (lidx) atomicAnd(reinterpret_cast<uint32_t *>(lidx) + (k >> 1), ~(0x8000u << (16 * (k & 1))));
+        }
     }
 }
 

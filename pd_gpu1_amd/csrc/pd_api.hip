@@ -57,6 +57,7 @@ void free_lds(pd_handle *h)
     dfree(h->lds.lidx);
     h->lds.ntiles = h->lds.ntiles_int = h->lds.max_staged = 0;
     h->lds_ok = false;
+    h->lidx_fail_ok = false;
 }
 
 void free_bonds(pd_handle *h)
@@ -138,7 +139,8 @@ int build_lds_tiles(pd_handle *h)
     if ((int64_t)mx * lds_record_bytes() > lds_bytes || mx > stage_cap()) return PD_E_CAPACITY;
     LAUNCH(hipMalloc(&T.stage_ptr, sizeof(int64_t) * (T.ntiles + 1)));
     LAUNCH(hipMalloc(&T.stage_idx, sizeof(int32_t) * (ptr[T.ntiles] > 0 ? ptr[T.ntiles] : 1)));
-    LAUNCH(hipMalloc(&T.lidx, sizeof(uint16_t) * (h->Nb > 0 ? h->Nb : 1)));
+    // an even count: breaks clear bit 15 with 32-bit atomics on word k/2
+    LAUNCH(hipMalloc(&T.lidx, sizeof(uint16_t) * (size_t)((h->Nb + 2) & ~(int64_t)1)));
     LAUNCH(hipMemcpyAsync(T.stage_ptr, ptr.data(), sizeof(int64_t) * (T.ntiles + 1), hipMemcpyHostToDevice, s));
     LAUNCH(launch_stage_tiles(s, 1, T.ntiles, T.tile_start, h->order, h->row_ptr, h->col, nullptr, T.stage_ptr,
                               T.stage_idx, T.lidx));
@@ -147,6 +149,7 @@ int build_lds_tiles(pd_handle *h)
     T.block = h->lds_block;
     T.block2 = h->lds_block2;
     h->lds_ok = true;
+    h->lidx_fail_ok = false;
     return PD_OK;
 }
 
@@ -302,6 +305,20 @@ hipError_t sync_rec_u(pd_handle *h)
     return e;
 }
 
+// LDS tiles: copy the fail bits into bit 15 of the local index words when
+// some call changed them other than through the LDS kernels' own breaks.
+// Like sync_rec_u it runs before a graph is captured or replayed.
+hipError_t sync_lidx_fail(pd_handle *h)
+{
+    if (!h->lds_ok || h->lidx_fail_ok) return hipSuccess;
+    hipError_t e = launch_lidx_sync(h->stream, h->Nb, h->fail, h->lds.lidx);
+    if (e == hipSuccess) h->lidx_fail_ok = true;
+    return e;
+}
+
+// Every change of the fail words outside the LDS kernels goes through here.
+void fail_changed(pd_handle *h) { h->lidx_fail_ok = false; }
+
 double *rec_or_null(const pd_handle *h) { return h->P.precision == PD_PREC_MIXED ? h->rec : nullptr; }
 
 // Force evaluation (PMB, or LPS theta pass + force pass) on the handle's
@@ -316,10 +333,13 @@ int force_launch(pd_handle *h, const int *stop)
     if (P.model == PD_MODEL_PMB) {
         // all rows i < N (Force_Bond.cu:169)
         if (P.precision == PD_PREC_FP64) {
+            fail_changed(h);
             LAUNCH(launch_pmb_fp64(s, stop, P.dim, h->N, h->row_ptr, h->col, h->kappa, h->idist, h->fail,
                                    cd3(h->x), cd3(h->u), h->f, P.s0, ymin, fp64_group(h)));
         } else {
             LAUNCH(sync_rec_u(h));
+            LAUNCH(sync_lidx_fail(h));
+            if (!h->lds_ok) fail_changed(h);
             if (h->lds_ok)
                 LAUNCH(launch_pmb_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, h->fail,
                                       h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
@@ -330,6 +350,7 @@ int force_launch(pd_handle *h, const int *stop)
     } else {
         // theta for i < N (Force_State.cu:339), force for i < N_int (:359)
         if (P.precision == PD_PREC_FP64) {
+            fail_changed(h);
             LAUNCH(launch_lps_theta_fp64(s, stop, P.dim, h->N, h->row_ptr, h->col, cd3(h->x), cd3(h->u), h->idist,
                                          h->fac, h->w, h->m, h->fail, P.vol, P.pratio, P.theta_skip_broken, h->e,
                                          h->theta));
@@ -341,9 +362,11 @@ int force_launch(pd_handle *h, const int *stop)
             float ctheta, A, B;
             lps_mixed_consts(P, &ctheta, &A, &B);
             LAUNCH(sync_rec_u(h));
+            LAUNCH(sync_lidx_fail(h));
+            if (!h->lds_ok) fail_changed(h);
             if (h->lds_ok) {
                 LAUNCH(launch_lps_theta_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
-                                            h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
+                                            h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
                 LAUNCH(launch_lps_force_lds(s, stop, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr, h->bnd,
                                             h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32, gcp(h)));
             } else {
@@ -381,6 +404,17 @@ int contact_launch(pd_handle *h)
     const pd_params &P = h->P;
     LAUNCH(launch_contact(h->stream, h->red_blocks, h->red_threads, h->N_int, h->dmg, cd3(h->x), h->u, h->v,
                           h->cyl, P.dt, P.dens, P.vol, h->partials, h->S, rec_or_null(h)));
+    return PD_OK;
+}
+
+// Force of one step, half 2 of it and half 1 of the next (no cylinder).
+int fused_step_launch(pd_handle *h)
+{
+    int rc = force_launch(h, nullptr);
+    if (rc) return rc;
+    const pd_params &P = h->P;
+    LAUNCH(launch_verlet21(h->stream, P.dim, h->N_int, P.n_top, P.n_bot, P.dt, P.dens, P.strip_velocity, h->v,
+                           cd3(h->f), cf3(h->f32), use32(h), cd3(h->b), h->u, rec_or_null(h)));
     return PD_OK;
 }
 
@@ -739,6 +773,7 @@ int pd_build_neighbors(pd_handle *h)
     if (rc) return rc;
     const int64_t nb = h->Nb;
     LAUNCH(dalloc(&h->fail, (size_t)fail_words(h)));
+    fail_changed(h);
     LAUNCH(launch_fill_u32(h->stream, h->fail, fail_words(h), 0xFFFFFFFFu));
     LAUNCH(dalloc(&h->idist, nb));
     LAUNCH(dalloc(&h->fac, nb));
@@ -812,6 +847,7 @@ int pd_init_state(pd_handle *h, int mode)
     h->rec_u_valid = false;
     REQUIRE(h->col, "pd_build_neighbors first");
     hipStream_t s = h->stream;
+    fail_changed(h);
     LAUNCH(launch_fill_u32(s, h->fail, fail_words(h), 0xFFFFFFFFu));
     D3 *sets[] = {&h->u, &h->v, &h->a, &h->b, &h->f, &h->vh, &h->vho, &h->fo};
     for (D3 *st : sets) LAUNCH(launch_fill_f64(s, st->p[0], 3 * (int64_t)h->N, 0.0));
@@ -831,6 +867,7 @@ int pd_set_crack_2d(pd_handle *h, double clength, double loc_x, double loc_y, do
     const double sq1 = std::sqrt(1 + std::tan(theta) * std::tan(theta));
     const double half_s = clength * std::sin(theta) / 2.0;
     const int vertical = std::fabs(theta - kPi / 2.0) < 0.0001;
+    fail_changed(h);
     LAUNCH(launch_crack_2d(h->stream, h->N_int, h->row_ptr, h->col, cd3(h->x), clength, loc_x, loc_y, vertical, t,
                            sq1, half_s, h->fail));
     return PD_OK;
@@ -840,6 +877,7 @@ int pd_set_notches_kw(pd_handle *h, double rad)
 {
     if (!h) return PD_E_INVALID;
     REQUIRE(h->col, "pd_build_neighbors first");
+    fail_changed(h);
     LAUNCH(launch_crack_kw(h->stream, h->N_int, h->row_ptr, h->col, cd3(h->x), rad, h->fail));
     return PD_OK;
 }
@@ -881,6 +919,8 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
     const pd_params &P = h->P;
     const double ymin = P.dim == 3 ? P.break_y_min : -INFINITY;
     double acc[2] = {0.0, 0.0};
+    LAUNCH(sync_lidx_fail(h));
+    if (!h->lds_ok || P.precision == PD_PREC_FP64) fail_changed(h);
     for (int r = 0; r < reps && rc == PD_OK; ++r) {
         if (P.precision == PD_PREC_MIXED) LAUNCH(launch_pack_u(h->stream, nullptr, P.dim, h->N, cd3(h->u), h->rec));
         LAUNCH(hipEventRecord(e[0], h->stream));
@@ -912,8 +952,8 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
             lps_mixed_consts(P, &ctheta, &A, &B);
             if (h->lds_ok)
                 LAUNCH(launch_lps_theta_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr,
-                                            h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
-                                            h->theta32, gcp(h)));
+                                            h->bnd, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32,
+                                            gcp(h)));
             else
                 LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, h->order, P.dim, g, h->N, h->row_ptr, h->col,
                                               h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
@@ -1133,10 +1173,24 @@ int pd_run_verlet(pd_handle *h, int nsteps)
     int rc = ensure_coef(h);
     if (rc) return rc;
     LAUNCH(sync_rec_u(h));
-    const int kind = 1 + (h->have_cyl ? 1 : 0);
-    rc = capture(h, kind, 1, [&]() { return step_launch(h); });
-    if (rc) return rc;
-    for (int s = 0; s < nsteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+    LAUNCH(sync_lidx_fail(h));
+    if (h->have_cyl || nsteps == 1) {
+        const int kind = 1 + (h->have_cyl ? 1 : 0);
+        rc = capture(h, kind, 1, [&]() { return step_launch(h); });
+        if (rc) return rc;
+        for (int s = 0; s < nsteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+    } else {
+        // half 1, then (force, half 2 + half 1 of the next step) from a
+        // graph, and the last step's force and half 2
+        rc = verlet1_launch(h);
+        if (rc) return rc;
+        rc = capture(h, 4, 1, [&]() { return fused_step_launch(h); });
+        if (rc) return rc;
+        for (int s = 0; s < nsteps - 1; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+        rc = force_launch(h, nullptr);
+        if (!rc) rc = verlet2_launch(h);
+        if (rc) return rc;
+    }
     LAUNCH(hipStreamSynchronize(h->stream));
     return PD_OK;
 }
@@ -1186,6 +1240,7 @@ int pd_adr_solve(pd_handle *h, int ct0, int max_iters, double tol, int *iters_ou
     int rc = ensure_coef(h);
     if (rc) return rc;
     LAUNCH(sync_rec_u(h));
+    LAUNCH(sync_lidx_fail(h));
     LAUNCH(launch_set_ct(h->stream, h->S, ct0, 1));
     const int chunk = 32;
     int launched = 0;
@@ -1357,6 +1412,7 @@ int pd_set_fail_bits(pd_handle *h, const uint32_t *in)
     if (!h || !in) return PD_E_INVALID;
     REQUIRE(h->fail, "pd_build_neighbors first");
     LAUNCH(hipStreamSynchronize(h->stream));
+    fail_changed(h);
     LAUNCH(hipMemcpy(h->fail, in, sizeof(uint32_t) * fail_words(h), hipMemcpyHostToDevice));
     return PD_OK;
 }

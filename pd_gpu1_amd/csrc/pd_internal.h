@@ -49,7 +49,8 @@ struct LdsTiles {
     int32_t *tile_start = nullptr;   // [ntiles + 1] row ranges of order[]
     int64_t *stage_ptr = nullptr;    // [ntiles + 1] ranges of stage_idx
     int32_t *stage_idx = nullptr;    // staged particle indices, ascending within a tile
-    uint16_t *lidx = nullptr;        // per bond: position of col[k] in its tile's staged list
+    uint16_t *lidx = nullptr;        // per bond: position of col[k] in its tile's staged list (bits 0-14)
+                                     // and a copy of its fail bit (bit 15)
 };
 
 struct pd_handle {
@@ -72,6 +73,7 @@ struct pd_handle {
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
     int lds_mode = -1;            // LDS-staged tiles: -1 where they fit (default), 0 off, 1 required
     bool lds_ok = false;          // the LDS tile lists are built and in use
+    bool lidx_fail_ok = false;    // bit 15 of every lidx word equals the bond's fail bit
     int lds_block = 512;          // threads per workgroup of the LDS kernels (pass 0)
     int lds_block2 = 512;         // ... of the LPS force pass
     int fp64_group = -1;          // lanes per row of the FP64 PMB kernel (1 = thread per row, -1 = by row length)
@@ -192,6 +194,7 @@ hipError_t launch_geom_check(hipStream_t s, int dim, int N, const int32_t *rp, c
 // k_force_lds.hip
 int stage_cap();
 int lds_record_bytes();
+hipError_t launch_lidx_sync(hipStream_t s, int64_t nb, const uint32_t *fail, uint16_t *lidx);
 hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_t *tile_start, const int32_t *order,
                               const int32_t *rp, const int32_t *col, int32_t *count, const int64_t *stage_ptr,
                               int32_t *stage_idx, uint16_t *lidx);
@@ -199,8 +202,8 @@ hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int
                           const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
                           double window, double ymin, const GeomCoef *gc);
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
-                                const int32_t *order, const int32_t *rp, const float2 *bnd, const uint32_t *fail,
-                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
+                                const int32_t *order, const int32_t *rp, const float2 *bnd, double *rec,
+                                const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
                                 const double *rec, float A, float B, double s0, double window, double ymin, F3 f, const GeomCoef *gc);
@@ -218,6 +221,8 @@ hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t 
 // k_integrate.hip
 hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
                           D3 v, CD3 a, D3 u, double *cyl_state, double *rec);
+hipError_t launch_verlet21(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double dens,
+                           double load, D3 v, CD3 f, CF3 f32, int use32, CD3 b, D3 u, double *rec);
 hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f,
                           CF3 f32, int use32, CD3 b);
 hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, int N_int, int N,

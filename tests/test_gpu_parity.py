@@ -338,8 +338,8 @@ def test_mixed_records_follow_every_displacement_update():
                 b.verlet_half1()
                 b.force()
                 b.verlet_half2()
-            assert np.array_equal(a.get("U"), b.get("U"))
-            assert np.array_equal(a.get("F"), b.get("F"))
+            for field in ("U", "V", "A", "F"):  # run_verlet fuses half 2 with the next half 1
+                assert np.array_equal(a.get(field), b.get(field)), field
             assert np.array_equal(a.fail_mask(), b.fail_mask())
         a.strip_displace(1e-6, -1e-6)
         uf, mask = a.get("U"), a.fail_mask()  # bonds that break in this pass still act in it
@@ -386,6 +386,46 @@ def test_lds_tiles_match_gathering_kernels(case):
     assert np.array_equal(f0, f1), float(np.max(np.abs(f0 - f1)) / np.max(np.abs(f0)))
     if th0 is not None:
         assert np.array_equal(th0, th1)
+
+
+@pytest.mark.parametrize("case", ["lps3d", "pmb2d_strips"])
+def test_lds_intact_bits_follow_breaks_and_fail_writes(case):
+    # The LDS kernels read a bond's intact flag from bit 15 of its local
+    # index word.  Breaks inside captured Verlet steps, a fail-mask upload,
+    # init_state and a crack must all leave the LDS path bitwise equal to
+    # the gathering kernels, which read the fail words themselves.
+    if case == "lps3d":
+        sc = scenarios.c2_lps_block(nx=24, ny=20, nz=16, precision=_lib.PD_PREC_MIXED)
+        u = np.stack([1.1e-3 * sc.x, 3e-4 * sc.y, 2.05e-3 * sc.z])
+        tile = 6
+    else:
+        sc = scenarios.c0_plate_pmb(scale=0.6)
+        u = np.stack([4e-3 * sc.x, 4.1e-3 * sc.y])
+        tile = 16
+    outs = []
+    for lds in (0, 1):
+        with sc.make(precision=_lib.PD_PREC_MIXED) as prob:
+            prob.set_lds(lds)
+            prob.set_tile(tile)
+            prob.set("U", u)
+            prob.run_verlet(6)
+            broken_by_steps = int((prob.fail_mask() == 0).sum())
+            m = prob.fail_mask()
+            m[3::7] = 0
+            prob.set_fail_mask(m)
+            prob.run_verlet(3)
+            f1, u1, m1 = prob.get("F"), prob.get("U"), prob.fail_mask()
+            prob.init_state()
+            if sc.p.dim == 2:
+                prob.set_crack_2d(0.01, 0.0, 0.0, 0.0)
+            prob.set("U", 0.5 * u)
+            prob.force()
+            outs.append((f1, u1, m1, prob.get("F"), prob.fail_mask(), prob.lds_info()[0], broken_by_steps))
+    g, l = outs
+    assert not g[5] and l[5]
+    assert g[6] == l[6] > 0
+    for a, b in zip(g[:5], l[:5]):
+        assert np.array_equal(a, b)
 
 
 def test_lds_capacity_is_reported_and_default_falls_back():
