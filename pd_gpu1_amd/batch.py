@@ -133,3 +133,52 @@ class Batch:
                                        fw.ctypes.data_as(_lib.U32)))
         mask = np.unpackbits(fw.view(np.uint8), bitorder="little")[:nb].astype(np.uint8)
         return rp, col, mask
+
+
+def main(argv=None):
+    """Run load increments of the seeded plate family and print one line per
+    increment (wall time, iterations) and a JSON summary:
+
+        python -m pd_gpu1_amd.batch --plates 512 --increments 2 --max-iters 6000
+    """
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description="batched ADR over the seeded plate family")
+    ap.add_argument("--plates", type=int, default=512)
+    ap.add_argument("--increments", type=int, default=1)
+    ap.add_argument("--max-iters", type=int, default=6000)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--du", type=float, default=1e-6)
+    ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--grid", type=int, default=0, help="workgroups (0: as many as are resident at once)")
+    ap.add_argument("--group", type=int, default=4)
+    ap.add_argument("--warm", type=int, default=0)
+    ap.add_argument("--retire", type=int, default=1)
+    a = ap.parse_args(argv)
+    specs = plate_family(a.plates)
+    rows = []
+    t0 = time.perf_counter()
+    with Batch(plate_params(), specs) as b:
+        b.set_group(a.group)
+        b.set_warm(a.warm)
+        b.set_retire(a.retire)
+        n_part, n_bonds, _ = b.info()
+        print(f"plates {a.plates} particles {n_part} bonds {n_bonds}", flush=True)
+        for k in range(a.increments):
+            its, res, ms = b.increment(a.du, a.tol, a.max_iters, block=a.block, grid=a.grid)
+            act = its > 0
+            rows.append({"increment": k, "ms": round(ms, 1), "active": int(act.sum()),
+                         "iterations_median": int(np.median(its[act])) if act.any() else 0,
+                         "iterations_max": int(its.max()), "unconverged": int(((res > a.tol) & act).sum()),
+                         "specimen_iterations": int(its.sum())})
+            print("increment %(increment)d: %(ms).1f ms, %(active)d active, iterations median "
+                  "%(iterations_median)d max %(iterations_max)d, unconverged %(unconverged)d" % rows[-1], flush=True)
+    print(json.dumps({"plates": a.plates, "particles": n_part, "bonds": n_bonds, "block": a.block,
+                      "group": a.group, "warm": a.warm, "total_s": sum(r["ms"] for r in rows) / 1e3,
+                      "wall_s": time.perf_counter() - t0, "increments": rows}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

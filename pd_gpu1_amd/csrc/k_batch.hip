@@ -2,8 +2,10 @@
 // plates relaxed by adaptive dynamic relaxation (ADR) in one persistent launch.
 //
 // One workgroup owns one specimen at a time and pulls the next specimen index
-// from a device work queue (an atomic counter), so 512 plates of 1.6k-40k
-// particles load-balance over the 256 CUs without host involvement.  Inside a
+// from a device work queue (an atomic counter over the specimens sorted by
+// bond count, largest first), so 512 plates of 1.6k-40k particles
+// load-balance over the 256 CUs without host involvement and the largest
+// plates, which set the wall time, start first.  Inside a
 // specimen the workgroup runs whole ADR iterations: PMB force over every row
 // (Force_Bond.cu:159-202 arithmetic, FP64, rows in CSR order), the damping
 // terms of kernel_cnn (Integrate.cu:14-34) and the residual reduced with the
@@ -42,21 +44,34 @@ __device__ __forceinline__ void block_canon4(double p[4], double (*sh)[BT / 64],
 }
 
 // Pair force term of bond k of a row at (x0+u0, x1+u1), Force_Bond.cu:185-196
-// operand order; breaks the bond when its stretch exceeds s0.
-__device__ __forceinline__ void bond_term(int64_t k, int c, double x0, double x1, double u0, double u1,
-                                          const double *X, const double *Y, const double *UX, const double *UY,
-                                          const double *kap, const double *idist, double s0, uint32_t *fail,
-                                          double &tx, double &ty)
+// operand order; breaks the bond when its stretch exceeds s0.  P[c] holds
+// (X[c] + UX[c], Y[c] + UY[c]), the first sum of that order, so the
+// neighbour costs one 16-byte load.
+__device__ __forceinline__ void bond_term(int64_t k, int c, const double2 *KD, double x0, double x1, double u0,
+                                          double u1, const double2 *P, double s0, uint32_t *fail, double &tx,
+                                          double &ty)
 {
-    double d0 = X[c] + UX[c] - x0 - u0;
-    double d1 = Y[c] + UY[c] - x1 - u1;
+    const double2 pc = P[c];
+    const double2 kd = KD[k];
+    double d0 = pc.x - x0 - u0;
+    double d1 = pc.y - x1 - u1;
     double nl = sqrt(d0 * d0 + d1 * d1);
-    double id = idist[k];
+    double id = kd.y;
     double sv = (nl - id) / id;
-    double ks = kap[k] * sv;
+    double ks = kd.x * sv;
     tx = ks * d0 / nl;
     ty = ks * d1 / nl;
     if (sv > s0) atomicAnd(&fail[k >> 5], ~(1u << (k & 31)));
+}
+
+// Workgroup-uniform values read from LDS, moved to scalar registers so that
+// the specimen's base pointers and loop bounds stay out of vector registers.
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double uniform_double(double v)
+{
+    const int2 w = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y)));
 }
 
 template <int BT, int G>
@@ -68,9 +83,12 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
     __shared__ double c_sh, r_sh;
     const int tid = threadIdx.x;
     for (;;) {
-        if (tid == 0) spec_sh = (int)atomicAdd(queue, 1u);
+        if (tid == 0) {
+            const int t = (int)atomicAdd(queue, 1u);
+            spec_sh = t < B.nspec ? B.order[t] : B.nspec;
+        }
         __syncthreads();
-        const int s = spec_sh;
+        const int s = uniform_int(spec_sh);
         __syncthreads();
         if (s >= B.nspec) break;
         if (B.retired && B.retired[s]) {
@@ -88,10 +106,11 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         double *FOX = B.fox + d.p_off, *FOY = B.foy + d.p_off;
         const int32_t *rp = B.row_ptr + d.rp_off;
         const int32_t *col = B.col + d.b_off;
-        const double *kap = B.kappa + d.b_off, *idist = B.idist + d.b_off;
+        const double2 *KD = B.kd + d.b_off;
         uint32_t *fail = B.fail + (d.b_off >> 5);
         const double mass = B.mass, dt = B.dt, s0 = B.s0;
         double *DUPX = B.dupx + d.p_off, *DUPY = B.dupy + d.p_off;
+        double2 *P = B.pos + d.p_off;
         if (B.warm) {
             // start from u + the previous increment; keep the start state
             for (int i = tid; i < Ni; i += BT) {
@@ -105,6 +124,8 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         // displacement control of the strips (Solve_Plate_Crack_Grow.cu:20-27 layout)
         for (int k = tid; k < d.n_top + d.n_bot; k += BT) UY[Ni + k] += k < d.n_top ? du : -du;
         __syncthreads();
+        for (int i = tid; i < N; i += BT) P[i] = make_double2(X[i] + UX[i], Y[i] + UY[i]);
+        __syncthreads();
         int it = 0;
         double r = 0.0;
         for (;;) {
@@ -116,7 +137,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                     for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
                         if (!intact(fail, k)) continue;
                         double tx, ty;
-                        bond_term(k, col[k], x0, x1, u0, u1, X, Y, UX, UY, kap, idist, s0, fail, tx, ty);
+                        bond_term(k, col[k], KD, x0, x1, u0, u1, P, s0, fail, tx, ty);
                         fx += tx;
                         fy += ty;
                     }
@@ -129,23 +150,31 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                     const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
                     double fx = 0.0, fy = 0.0;
                     const int32_t kb = rp[i], ke = rp[i + 1];
+                    // the neighbour index and intact bit of the lane's next
+                    // bond load while the current bond computes (a bond's
+                    // bit changes only when that bond itself breaks)
+                    int32_t k = kb + lane;
+                    int c = k < ke ? col[k] : 0;
+                    int ok = k < ke && intact(fail, k);
                     for (int32_t k0 = kb; k0 < ke; k0 += G) {
-                        const int32_t k = k0 + lane;
+                        const int32_t kn = k + G;
+                        const int cn = kn < ke ? col[kn] : 0;
+                        const int okn = kn < ke && intact(fail, kn);
                         double tx = 0.0, ty = 0.0;
-                        int ok = 0;
-                        if (k < ke && intact(fail, k)) {
-                            ok = 1;
-                            bond_term(k, col[k], x0, x1, u0, u1, X, Y, UX, UY, kap, idist, s0, fail, tx, ty);
-                        }
+                        if (ok) bond_term(k, c, KD, x0, x1, u0, u1, P, s0, fail, tx, ty);
                         // add the terms in bond order, as the sequential loop does
                         const int cnt = ke - k0 < G ? ke - k0 : G;
-                        for (int l = 0; l < cnt; ++l) {
+#pragma unroll
+                        for (int l = 0; l < G; ++l) {
                             const double sx = __shfl(tx, l, G), sy = __shfl(ty, l, G);
-                            if (__shfl(ok, l, G)) {
+                            if (l < cnt && __shfl(ok, l, G)) {
                                 fx += sx;
                                 fy += sy;
                             }
                         }
+                        k = kn;
+                        c = cn;
+                        ok = okn;
                     }
                     if (lane == 0) {
                         FX[i] = fx;
@@ -192,9 +221,9 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                 r_sh = tot[3] > 0.0 ? sqrt(tot[2]) / sqrt(tot[3]) : (tot[2] > 0.0 ? INFINITY : 0.0);
             }
             __syncthreads();
-            r = r_sh;
+            r = uniform_double(r_sh);
             if (r <= tol || it >= max_iters || !(r == r)) break;
-            const double cc = c_sh;
+            const double cc = uniform_double(c_sh);
             for (int i = tid; i < Ni; i += BT) {
                 double h0, h1;
                 const double fx = FX[i], fy = FY[i];
@@ -209,8 +238,10 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                 VHY[i] = h1;
                 VX[i] = 0.5 * (VHOX[i] + h0);
                 VY[i] = 0.5 * (VHOY[i] + h1);
-                UX[i] += h0 * dt;
-                UY[i] += h1 * dt;
+                const double ux1 = UX[i] + h0 * dt, uy1 = UY[i] + h1 * dt;
+                UX[i] = ux1;
+                UY[i] = uy1;
+                P[i] = make_double2(X[i] + ux1, Y[i] + uy1);
                 VHOX[i] = h0;
                 VHOY[i] = h1;
                 FOX[i] = fx;
@@ -237,6 +268,13 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
 } // namespace
 
 namespace {
+__global__ void k_pack_bond_pairs(int64_t nb, const double *__restrict__ kappa, const double *__restrict__ idist,
+                                  double2 *kd)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nb) kd[k] = make_double2(kappa[k], idist[k]);
+}
+
 template <int BT>
 hipError_t batch_g(hipStream_t st, int grid, int group, const BatchView &B, double du, double tol, int max_iters,
                    unsigned *queue, int32_t *iters_out, double *resid_out)
@@ -252,6 +290,29 @@ hipError_t batch_g(hipStream_t st, int grid, int group, const BatchView &B, doub
     return hipGetLastError();
 }
 } // namespace
+
+hipError_t launch_pack_bond_pairs(hipStream_t st, int64_t nb, const double *kappa, const double *idist,
+                                  double2 *kd)
+{
+    if (nb <= 0) return hipSuccess;
+    k_pack_bond_pairs<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(nb, kappa, idist, kd);
+    return hipGetLastError();
+}
+
+// Workgroups of k_batch_adr<block, group> that fit on one CU at once.
+int batch_resident_per_cu(int block, int group)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define PD_OCC(BT, GV)                                                                                      \
+    if (block == BT && group == GV)                                                                     \
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(k_batch_adr<BT, GV>), BT, 0);
+    PD_OCC(256, 1) PD_OCC(256, 4) PD_OCC(256, 8) PD_OCC(256, 16) PD_OCC(256, 32)
+    PD_OCC(512, 1) PD_OCC(512, 4) PD_OCC(512, 8) PD_OCC(512, 16) PD_OCC(512, 32)
+    PD_OCC(1024, 1) PD_OCC(1024, 4) PD_OCC(1024, 8) PD_OCC(1024, 16) PD_OCC(1024, 32)
+#undef PD_OCC
+    return e == hipSuccess && n > 0 ? n : 1;
+}
 
 hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
                             double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out)

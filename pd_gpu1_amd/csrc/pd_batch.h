@@ -13,16 +13,21 @@ struct BatchSpec {
 struct BatchView {
     int nspec;
     const BatchSpec *spec;
+    const int32_t *order;  // queue ticket -> specimen, largest bond count first
     const double *x, *y;
     double *ux, *uy, *fx, *fy, *vx, *vy, *vhx, *vhy, *vhox, *vhoy, *fox, *foy;
     double *dupx, *dupy;  // previous increment of u (warm start), then the start state
+    double2 *pos;         // per particle (x + ux, y + uy): the one 16-byte gather of a bond
     int warm;             // 1: start each increment from u + previous increment
     int32_t *retired;     // non-null: skip specimens marked 1, mark those that end unconverged
     const int32_t *row_ptr, *col;
-    const double *kappa, *idist;
+    const double2 *kd;    // per bond {kappa, |xi|}
     uint32_t *fail;
     double mass, dt, s0;
 };
 
+hipError_t launch_pack_bond_pairs(hipStream_t st, int64_t nb, const double *kappa, const double *idist,
+                                  double2 *kd);
+int batch_resident_per_cu(int block, int group);
 hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
                             double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out);

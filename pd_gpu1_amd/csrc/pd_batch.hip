@@ -8,6 +8,7 @@
 // 32-bond boundaries so fail words copy as whole words.
 #include "pd_batch.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -18,10 +19,12 @@ struct pd_batch {
     std::string err;
     std::vector<BatchSpec> spec;
     BatchSpec *d_spec = nullptr;
+    int32_t *d_order = nullptr;   // queue order: specimens by bond count, largest first
     int64_t n_total = 0, nb_total = 0, rp_total = 0;
     double *buf = nullptr;        // 16 particle fields of n_total doubles
+    double2 *pos = nullptr;       // current positions, refreshed by the kernel
     int32_t *row_ptr = nullptr, *col = nullptr;
-    double *kappa = nullptr, *idist = nullptr;
+    double2 *kd = nullptr;        // per bond {kappa, |xi|}
     uint32_t *fail = nullptr;
     unsigned *queue = nullptr;
     int32_t *d_iters = nullptr;
@@ -36,15 +39,17 @@ struct pd_batch {
         BatchView v{};
         v.nspec = (int)spec.size();
         v.spec = d_spec;
+        v.order = d_order;
         double *f[16];
         for (int k = 0; k < 16; k++) f[k] = buf + (size_t)k * n_total;
         v.x = f[0]; v.y = f[1]; v.ux = f[2]; v.uy = f[3]; v.fx = f[4]; v.fy = f[5];
         v.vx = f[6]; v.vy = f[7]; v.vhx = f[8]; v.vhy = f[9]; v.vhox = f[10]; v.vhoy = f[11];
         v.fox = f[12]; v.foy = f[13];
         v.dupx = f[14]; v.dupy = f[15];
+        v.pos = pos;
         v.warm = warm;
         v.retired = retire ? d_retired : nullptr;
-        v.row_ptr = row_ptr; v.col = col; v.kappa = kappa; v.idist = idist; v.fail = fail;
+        v.row_ptr = row_ptr; v.col = col; v.kd = kd; v.fail = fail;
         v.mass = P.mass; v.dt = P.dt; v.s0 = P.s0;
         return v;
     }
@@ -61,7 +66,7 @@ struct pd_batch {
 
 static void batch_free(pd_batch *b)
 {
-    void *ptrs[] = {b->d_spec, b->buf, b->row_ptr, b->col, b->kappa, b->idist, b->fail, b->queue, b->d_iters,
+    void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->fail, b->queue, b->d_iters,
                     b->d_resid, b->d_retired};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -127,12 +132,13 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
         return hipMalloc(p, bytes ? bytes : 16) == hipSuccess ? PD_OK : PD_E_HIP;
     };
     if (!rc) rc = alloc((void **)&b->buf, sizeof(double) * 16 * b->n_total);
+    if (!rc) rc = alloc((void **)&b->pos, sizeof(double2) * b->n_total);
     if (!rc) rc = alloc((void **)&b->row_ptr, sizeof(int32_t) * b->rp_total);
     if (!rc) rc = alloc((void **)&b->col, sizeof(int32_t) * b->nb_total);
-    if (!rc) rc = alloc((void **)&b->kappa, sizeof(double) * b->nb_total);
-    if (!rc) rc = alloc((void **)&b->idist, sizeof(double) * b->nb_total);
+    if (!rc) rc = alloc((void **)&b->kd, sizeof(double2) * b->nb_total);
     if (!rc) rc = alloc((void **)&b->fail, sizeof(uint32_t) * (b->nb_total / 32 + 1));
     if (!rc) rc = alloc((void **)&b->d_spec, sizeof(BatchSpec) * nspec);
+    if (!rc) rc = alloc((void **)&b->d_order, sizeof(int32_t) * nspec);
     if (!rc) rc = alloc((void **)&b->queue, sizeof(unsigned));
     if (!rc) rc = alloc((void **)&b->d_iters, sizeof(int32_t) * nspec);
     if (!rc) rc = alloc((void **)&b->d_resid, sizeof(double) * nspec);
@@ -163,8 +169,8 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
         cp(b->buf + 1 * (size_t)b->n_total + d.p_off, h->x.p[1], sizeof(double) * d.n);
         cp(b->row_ptr + d.rp_off, h->row_ptr, sizeof(int32_t) * (d.n + 1));
         cp(b->col + d.b_off, h->col, sizeof(int32_t) * d.nb);
-        cp(b->kappa + d.b_off, h->kappa, sizeof(double) * d.nb);
-        cp(b->idist + d.b_off, h->idist, sizeof(double) * d.nb);
+        if (e == hipSuccess) e = launch_pack_bond_pairs(nullptr, d.nb, h->kappa, h->idist, b->kd + d.b_off);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
         cp(b->fail + d.b_off / 32, fw.data(), sizeof(uint32_t) * fw.size());
         if (e != hipSuccess) {
             b->err = std::string("batch copy: ") + hipGetErrorString(e);
@@ -173,6 +179,14 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
     }
     if (!rc && hipMemcpy(b->d_spec, b->spec.data(), sizeof(BatchSpec) * nspec, hipMemcpyHostToDevice) != hipSuccess)
         rc = PD_E_HIP;
+    if (!rc) {
+        // the largest plates set the wall time of an increment: hand them out first
+        std::vector<int32_t> ord((size_t)nspec);
+        for (int s = 0; s < nspec; ++s) ord[s] = s;
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) { return b->spec[a].nb > b->spec[c].nb; });
+        if (hipMemcpy(b->d_order, ord.data(), sizeof(int32_t) * nspec, hipMemcpyHostToDevice) != hipSuccess)
+            rc = PD_E_HIP;
+    }
     for (pd_handle *h : hs)
         if (h) pd_destroy(h);
     if (rc) {
@@ -225,9 +239,10 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     B_TRY(hipSetDevice(b->device));
     const int nspec = (int)b->spec.size();
     if (grid <= 0) {
+        // every workgroup resident at once (a persistent grid)
         int dev_cu = 0;
         B_TRY(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, b->device));
-        grid = dev_cu * (block == 1024 ? 2 : 4);
+        grid = dev_cu * batch_resident_per_cu(block, b->group);
     }
     if (grid > nspec) grid = nspec;
     hipEvent_t e0, e1;
