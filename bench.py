@@ -48,11 +48,12 @@ def lps_pass_bytes(nb, n, n_int, dim=3):
 
 def kernel_stream_bytes(nb, n, dim=3, lds=True, recomputed=True):
     """Bytes the LDS tile kernels themselves read per pass: per bond the
-    16-bit local index (2 B) and the fail bit (1/8 B), plus the 8-byte
+    16-bit local index word, which carries the fail bit (2 B; the gathering
+    kernels read col 4 B and the fail bit 1/8 B), plus the 8-byte
     coefficient pair when it is streamed rather than recomputed; per particle
     its own 64-byte record, row_ptr (4 B), the output (4D B) and the staged
     records (counted in the measured traffic, not here)."""
-    per_bond = (2.0 if lds else 4.0) + 0.125 + (0.0 if recomputed else 8.0)
+    per_bond = (2.0 if lds else 4.125) + (0.0 if recomputed else 8.0)
     return per_bond * nb + (64 + 4 + 4 * dim) * n
 
 
@@ -73,9 +74,29 @@ def load_traffic(kernel_key):
         return None
 
 
-def cpu_baseline(sample_dims=(200, 200, 100), evals=40):
-    """FP64 oracle (OpenMP) on a sub-block of the same lattice and physics."""
+def cpu_model():
+    """Host CPU model name and the logical CPUs this process may use."""
+    name = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"cpu_model": name, "logical_cpus": os.cpu_count(), "cpus_available": avail}
+
+
+def cpu_baseline(sample_dims=(200, 200, 100), evals=40, one_thread_evals=2):
+    """FP64 oracle (OpenMP) on a sub-block of the same lattice and physics,
+    with all threads and with one (the reference's sequential CPU path)."""
     import numpy as np
+    from oracle import oracle as O
     from oracle.runner import OracleModel
     from pd_gpu1_amd import scenarios
 
@@ -92,7 +113,13 @@ def cpu_baseline(sample_dims=(200, 200, 100), evals=40):
     for _ in range(evals):
         m.step()
     dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    threads = O.get_threads()
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    for _ in range(one_thread_evals):
+        m.step()
+    dt1 = time.perf_counter() - t0
+    O.set_threads(threads)
     del np
     return {
         "value": nb * evals / dt,
@@ -101,10 +128,37 @@ def cpu_baseline(sample_dims=(200, 200, 100), evals=40):
         "kind": "port",
         "sample": f"FP64 oracle, {nx}x{ny}x{nz} sub-block of the same jittered LPS lattice "
                   f"({len(sc.x)} particles, {nb} bonds), {evals} full Verlet+LPS steps, {dt:.2f} s",
+        "one_thread": {"value": nb * one_thread_evals / dt1, "unit": "bond-updates/s",
+                       "sample": f"the same, {one_thread_evals} steps on 1 thread, {dt1:.2f} s"},
+        **cpu_model(),
     }
 
 
-def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log=False, max_iters=12000):
+def batch_cpu_baseline(specs, du, tol, max_iters, T):
+    """FP64 oracle ADR, one increment, on the given specimens one after the
+    other (all OpenMP threads); the rate is bond-iterations per second."""
+    from oracle import oracle as O
+    from oracle.runner import adr_increment, plate_specimen
+    from pd_gpu1_amd import batch as PB
+    from pd_gpu1_amd import scenarios
+
+    sc = scenarios.params_dict(PB.plate_params())
+    work, its_all = 0, []
+    t = 0.0
+    for n, _, clen, ang in specs:
+        m = plate_specimen(sc, n, clen, ang)
+        t0 = time.perf_counter()
+        its, _ = adr_increment(m, du, tol, max_iters, T)
+        t += time.perf_counter() - t0
+        work += len(m.col) * (its + 1)  # force evaluations = iterations + the converged check
+        its_all.append(its)
+    return {"value": work / t, "unit": "bond-iterations/s", "cores": O.get_threads(), "kind": "port",
+            "sample": f"FP64 oracle ADR, increment 0 of the first {len(specs)} plates of the family "
+                      f"(sides {[int(s[0]) for s in specs]}), {sum(its_all)} iterations, {t:.2f} s"}
+
+
+def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log=False, max_iters=12000,
+                with_cpu=True):
     """Batched ADR timing and the warm-start comparison (see module doc)."""
     import numpy as np
     from pd_gpu1_amd import batch as PB
@@ -128,15 +182,21 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                 its, res, ms = b.increment(W.DU, 1e-6, max_iters, block=block)
                 active = its > 0
                 unconv = int(((res > 1e-6) & active).sum())
+                nbs = [b.spec_info(s_)[2] for s_ in range(len(specs))] if k == 0 else None
                 incs.append({"wall_ms": round(ms, 1), "active": int(active.sum()),
                              "iterations_median": int(np.median(its[active])) if active.any() else 0,
                              "iterations_max": int(its.max()), "unconverged": unconv})
+                if nbs is not None:
+                    incs[-1]["bond_iterations"] = int(sum(nb * (int(i) + 1) for nb, i in zip(nbs, its)))
                 if log:
                     print(f"[bench] batch {'warm' if mode else 'plain'} increment {k}: {ms:.0f} ms, "
                           f"{int(active.sum())} active, iterations median {incs[-1]['iterations_median']} "
                           f"max {its.max()}, unconverged {unconv}", file=sys.stderr, flush=True)
             runs["warm" if mode else "plain"] = incs
     tot = {k: sum(i["wall_ms"] for i in v) for k, v in runs.items()}
+    # bond-iterations per second of increment 0 (plain), for the CPU comparison
+    gpu_rate = runs["plain"][0]["bond_iterations"] / (runs["plain"][0]["wall_ms"] / 1e3)
+    cpu = batch_cpu_baseline(specs[:16], W.DU, 1e-6, max_iters, block) if with_cpu else None
     out["batched"] = {"workload": f"{n_batch} PMB plates, sides 40..200 (seed 0x5EED0003+k), {batch_increments} "
                                   "increments of du=1e-6 m, ADR to relative residual 1e-6, one persistent launch "
                                   f"per increment; a specimen that has not converged after {max_iters} iterations "
@@ -144,7 +204,8 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                       "particles": n_part, "bonds": n_bonds, "plain_s": tot["plain"] / 1e3,
                       "warm_s": tot["warm"] / 1e3, "speedup": tot["plain"] / tot["warm"],
                       "increments_plain": runs["plain"], "increments_warm": runs["warm"],
-                      "launch_shape": {"block": block, "lanes_per_row": group}, "autotune_ms_per_iteration": tuned}
+                      "launch_shape": {"block": block, "lanes_per_row": group}, "autotune_ms_per_iteration": tuned,
+                      "increment0_bond_iterations_per_s": gpu_rate, "cpu_baseline": cpu}
     wpath = os.path.join(ROOT, "pd_gpu1_amd", "data", "mpnn_warmstart.npy")
     specs = W.eval_specs(n_eval)
     with Mpnn() as net:
@@ -375,8 +436,9 @@ def main():
         print(f"[bench] C0 and Kalthoff-Winkler runs {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_adr:
         t = time.perf_counter()
-        out["adr"] = (adr_section(n_batch=512, batch_increments=50, log=True, max_iters=6000) if args.c3_full
-                      else adr_section())
+        with_cpu = not args.no_cpu_baseline
+        out["adr"] = (adr_section(n_batch=512, batch_increments=50, log=True, max_iters=6000, with_cpu=with_cpu)
+                      if args.c3_full else adr_section(with_cpu=with_cpu))
         print(f"[bench] ADR measurements {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

@@ -83,6 +83,8 @@ def lib():
         L.orc_cylinder_half1.argtypes = [C.c_double, D, D, D]
         L.orc_contact.argtypes = [C.c_int, D, DPP, DPP, DPP, D, D, C.c_double, C.c_double, C.c_double, DPP]
         L.orc_cylinder_half2.argtypes = [C.c_double, C.c_double, D, D, D]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         _lib = L
     return _lib
 
@@ -313,3 +315,12 @@ def contact(N, dmg, X, U, V, cyl, cdisp, dt, dens, vol):
 
 def cylinder_half2(dt, mass, st):
     lib().orc_cylinder_half2(dt, mass, _p(st["vel"], D), _p(st["acc"], D), _p(st["force"], D))
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops (omp_set_num_threads)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())

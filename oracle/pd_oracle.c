@@ -18,6 +18,7 @@
  */
 #include "pd_oracle.h"
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -810,3 +811,11 @@ void orc_cylinder_half2(double dt, double mass, double *cvel, double *cacc, doub
         cforce[i] = 0.0;
     }
 }
+
+/* ------------------------------------------------------------- threading */
+void orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int orc_get_threads(void) { return omp_get_max_threads(); }

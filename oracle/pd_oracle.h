@@ -158,4 +158,8 @@ void orc_cylinder_half2(double dt, double mass, double *cvel, double *cacc, doub
 #ifdef __cplusplus
 }
 #endif
+/* Threads of the OpenMP loops (the CPU baseline times 1 thread and all). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 #endif

@@ -144,3 +144,43 @@ class OracleModel:
 
     def stack(self, name):
         return np.stack(getattr(self, name))
+
+
+# ------------------------------------------------------- batched-plate ADR
+def plate_specimen(sc, n, clen, ang):
+    """One plate of the batched family (Coord.cu:47-66 layout with 3 strip
+    rows above and below, centred crack of length clen at angle ang) set up
+    like pd_batch_create does: CSR, volume correction, intact bonds, crack.
+    sc: dict of model scalars (see OracleModel)."""
+    x, y = O.coord_plate_strips(n, n, sc["dx"])
+    sc = dict(sc)
+    sc["n_top"] = sc["n_bot"] = 3 * n
+    m = OracleModel(sc, x, y, None, n * n)
+    m.build_neighbors()
+    m.init_state()
+    if clen > 0:
+        m.set_crack_2d(clen, 0.0, 0.0, ang)
+    return m
+
+
+def adr_increment(m, du, tol, max_iters, T):
+    """One load increment of the batched ADR: strips move by +du / -du, then
+    ADR (Integrate.cu:7-129 with the canonical reduction tree over T lanes)
+    until the relative residual ||F+b|| free / ||F|| strips <= tol or
+    max_iters.  Returns (iterations, residual)."""
+    Ni, nt = m.N_int, m.sc["n_top"]
+    m.U[1][Ni:Ni + nt] += du
+    m.U[1][Ni + nt:] += -du
+    sc = m.sc
+    it = 0
+    while True:
+        m.force()
+        cn, disp = O.adr_terms(2, Ni, sc["mass"], sc["dt"], m.U, m.F, m.FO, m.VHO)
+        rn, rd = O.resid_terms(2, Ni, m.N, m.F, m.B)
+        c = O.adr_c(O.sum_tree(cn, T), O.sum_tree(disp, T))
+        rnum, rden = O.sum_tree(rn, T), O.sum_tree(rd, T)
+        r = np.sqrt(rnum) / np.sqrt(rden) if rden > 0 else (np.inf if rnum > 0 else 0.0)
+        if r <= tol or it >= max_iters or r != r:
+            return it, r
+        O.adr_integrate(2, Ni, it, c, sc["mass"], sc["dt"], m.F, m.B, m.VH, m.V, m.VHO, m.U, m.FO)
+        it += 1

@@ -225,7 +225,9 @@ __global__ __launch_bounds__(256) void k_lps_force_fp64(const int *__restrict__ 
     if (DIM == 3) f.p[2][i] = fz;
 }
 
-inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
+// at least one block: every kernel bounds-checks its index, so an empty
+// range (say N_int = 0) is a no-op launch rather than an invalid grid
+inline int nblk(int64_t n, int t) { return n > 0 ? (int)((n + t - 1) / t) : 1; }
 
 } // namespace
 

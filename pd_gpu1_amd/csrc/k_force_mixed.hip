@@ -222,7 +222,7 @@ __global__ void k_geom_check(int dim, int N, const int32_t *__restrict__ rp, con
 inline int grid8(int64_t threads)
 {
     int64_t b = (threads + 255) / 256;
-    return (int)((b + 7) / 8 * 8);
+    return b > 0 ? (int)((b + 7) / 8 * 8) : 8;  // the kernels bounds-check their rows
 }
 
 template <int DIM, int G, bool GC>

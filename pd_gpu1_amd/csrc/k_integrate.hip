@@ -356,7 +356,9 @@ __global__ void k_f32_to_f64(const float *a, double *b, int64_t n)
     if (i < n) b[i] = (double)a[i];
 }
 
-inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
+// at least one block: every kernel bounds-checks its index, so an empty
+// range (say N_int = 0) is a no-op launch rather than an invalid grid
+inline int nblk(int64_t n, int t) { return n > 0 ? (int)((n + t - 1) / t) : 1; }
 
 } // namespace
 

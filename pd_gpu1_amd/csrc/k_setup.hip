@@ -248,7 +248,9 @@ __global__ void k_copy16(const double2 *__restrict__ a, double2 *__restrict__ b,
     for (; t < n; t += (int64_t)gridDim.x * blockDim.x) b[t] = a[t];
 }
 
-inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
+// at least one block: every kernel bounds-checks its index, so an empty
+// range (say N_int = 0) is a no-op launch rather than an invalid grid
+inline int nblk(int64_t n, int t) { return n > 0 ? (int)((n + t - 1) / t) : 1; }
 
 } // namespace
 

@@ -5,41 +5,17 @@ import pytest
 
 from pd_gpu1_amd import batch as PB
 from pd_gpu1_amd import scenarios
-from oracle import oracle as O
-from oracle.runner import OracleModel
+from oracle.runner import adr_increment, plate_specimen
 
 pytestmark = pytest.mark.gpu
 
 
 def oracle_specimen(p, n, clen, ang):
-    x, y = O.coord_plate_strips(n, n, p.dx)
-    sc = scenarios.params_dict(p)
-    sc["n_top"] = sc["n_bot"] = 3 * n
-    m = OracleModel(sc, x, y, None, n * n)
-    m.build_neighbors()
-    m.init_state()
-    if clen > 0:
-        m.set_crack_2d(clen, 0.0, 0.0, ang)
-    return m
+    return plate_specimen(scenarios.params_dict(p), n, clen, ang)
 
 
 def oracle_increment(m, du, tol, max_iters, T):
-    Ni, nt = m.N_int, m.sc["n_top"]
-    m.U[1][Ni:Ni + nt] += du
-    m.U[1][Ni + nt:] += -du
-    sc = m.sc
-    it = 0
-    while True:
-        m.force()
-        cn, disp = O.adr_terms(2, Ni, sc["mass"], sc["dt"], m.U, m.F, m.FO, m.VHO)
-        rn, rd = O.resid_terms(2, Ni, m.N, m.F, m.B)
-        c = O.adr_c(O.sum_tree(cn, T), O.sum_tree(disp, T))
-        rnum, rden = O.sum_tree(rn, T), O.sum_tree(rd, T)
-        r = np.sqrt(rnum) / np.sqrt(rden) if rden > 0 else (np.inf if rnum > 0 else 0.0)
-        if r <= tol or it >= max_iters or r != r:
-            return it, r
-        O.adr_integrate(2, Ni, it, c, sc["mass"], sc["dt"], m.F, m.B, m.VH, m.V, m.VHO, m.U, m.FO)
-        it += 1
+    return adr_increment(m, du, tol, max_iters, T)
 
 
 @pytest.mark.parametrize("block,group", [(256, 1), (1024, 1), (256, 8), (1024, 32)])
