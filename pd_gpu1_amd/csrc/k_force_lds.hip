@@ -213,8 +213,28 @@ __device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_
     const int lane = threadIdx.x & (G - 1);                                                      \
     const int r1 = tile_start[t + 1];
 
-template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, int ntiles,
+// Verlet epilogue of a row whose force is final (FUSE kernels): lane d < DIM
+// of the row's group updates component d, ud being the row's u from its
+// record, with the arithmetic of k_verlet2 followed by k_verlet1, and writes
+// it to the next record buffer.  Rows past the integrated ones only move with
+// their strip (Solve_Plate_Crack_Grow.cu:20-27) or keep u.
+__device__ __forceinline__ void verlet_row(const VerletFuse &vf, int i, int d, float fd32, double ud)
+{
+    if (i < vf.N_int) {
+        double *vr = vf.vrec + (size_t)i * 8;
+        const double ad = ((double)fd32 + vr[3 + d]) / vf.dens;  // Integrate.cu:465-481
+        double vd = vr[d] + (vf.dt / 2.0) * ad;
+        vd = vd + (vf.dt / 2.0) * ad;                            // Integrate.cu:447-463
+        vr[d] = vd;
+        ud = ud + vd * vf.dt;
+    } else if (d == 1 && i < vf.N_int + vf.n_top + vf.n_bot) {
+        ud = i < vf.N_int + vf.n_top ? ud + vf.load * vf.dt : ud - vf.load * vf.dt;
+    }
+    vf.rec_next[(size_t)i * kRec + 4 + d] = ud;
+}
+
+template <int DIM, int G, bool GC, bool FUSE>
+__device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int ntiles,
                                                   const int32_t *__restrict__ tile_start,
                                                   const int32_t *__restrict__ order,
                                                   const int64_t *__restrict__ stage_ptr,
@@ -222,7 +242,7 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
                                                   const int32_t *__restrict__ rp, uint16_t *lidx,
                                                   const float2 *__restrict__ bnd, uint32_t *fail,
                                                   const double *__restrict__ rec, F3 f, double s0, float s0f,
-                                                  float win, double ymin, GeomCoef gc)
+                                                  float win, double ymin, GeomCoef gc, VerletFuse vf)
 {
     PD_TILE_PROLOGUE
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
@@ -251,6 +271,8 @@ __global__ __launch_bounds__(1024) void k_pmb_lds(const int *__restrict__ stop, 
             f.p[1][i] = sy;
             if (DIM == 3) f.p[2][i] = sz;
         }
+        if (FUSE && lane < DIM)
+            verlet_row(vf, i, lane, lane == 0 ? sx : lane == 1 ? sy : sz, lane == 0 ? ri.u0 : lane == 1 ? ri.u1 : ri.u2);
     }
 }
 
@@ -296,8 +318,8 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
     }
 }
 
-template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ stop, int ntiles,
+template <int DIM, int G, bool GC, bool FUSE>
+__device__ __forceinline__ void lps_force_lds_body(const int *__restrict__ stop, int ntiles,
                                                         const int32_t *__restrict__ tile_start,
                                                         const int32_t *__restrict__ order,
                                                         const int64_t *__restrict__ stage_ptr,
@@ -305,7 +327,8 @@ __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ 
                                                         const int32_t *__restrict__ rp, uint16_t *lidx,
                                                         const float2 *__restrict__ bnd, uint32_t *fail,
                                                         const double *__restrict__ rec, float A, float B,
-                                                        double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc)
+                                                        double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc,
+                                                        VerletFuse vf)
 {
     PD_TILE_PROLOGUE
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
@@ -336,8 +359,59 @@ __global__ __launch_bounds__(1024) void k_lps_force_lds(const int *__restrict__ 
             f.p[1][i] = sy;
             if (DIM == 3) f.p[2][i] = sz;
         }
+        if (FUSE && lane < DIM)
+            verlet_row(vf, i, lane, lane == 0 ? sx : lane == 1 ? sy : sz, lane == 0 ? ri.u0 : lane == 1 ? ri.u1 : ri.u2);
     }
 }
+
+// The force kernels, and their variants that also apply the Verlet epilogue
+// (separate names, so that profiles tell the two apart).
+#define PD_PMB_LDS_PARAMS                                                                                   \
+    const int *__restrict__ stop, int ntiles, const int32_t *__restrict__ tile_start,                      \
+        const int32_t *__restrict__ order, const int64_t *__restrict__ stage_ptr,                          \
+        const int32_t *__restrict__ stage_idx, const int32_t *__restrict__ rp, uint16_t *lidx,             \
+        const float2 *__restrict__ bnd, uint32_t *fail, const double *__restrict__ rec, F3 f, double s0,  \
+        float s0f, float win, double ymin, GeomCoef gc, VerletFuse vf
+#define PD_PMB_LDS_ARGS \
+    stop, ntiles, tile_start, order, stage_ptr, stage_idx, rp, lidx, bnd, fail, rec, f, s0, s0f, win, ymin, gc, vf
+#define PD_LPSF_LDS_PARAMS                                                                                  \
+    const int *__restrict__ stop, int ntiles, const int32_t *__restrict__ tile_start,                      \
+        const int32_t *__restrict__ order, const int64_t *__restrict__ stage_ptr,                          \
+        const int32_t *__restrict__ stage_idx, const int32_t *__restrict__ rp, uint16_t *lidx,             \
+        const float2 *__restrict__ bnd, uint32_t *fail, const double *__restrict__ rec, float A, float B,  \
+        double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc, VerletFuse vf
+#define PD_LPSF_LDS_ARGS \
+    stop, ntiles, tile_start, order, stage_ptr, stage_idx, rp, lidx, bnd, fail, rec, A, B, s0, s0f, win, ymin, f, gc, vf
+
+template <int DIM, int G, bool GC>
+__global__ __launch_bounds__(1024) void k_pmb_lds(PD_PMB_LDS_PARAMS)
+{
+    pmb_lds_body<DIM, G, GC, false>(PD_PMB_LDS_ARGS);
+}
+
+template <int DIM, int G, bool GC>
+__global__ __launch_bounds__(1024) void k_pmb_verlet_lds(PD_PMB_LDS_PARAMS)
+{
+    pmb_lds_body<DIM, G, GC, true>(PD_PMB_LDS_ARGS);
+}
+
+template <int DIM, int G, bool GC>
+__global__ __launch_bounds__(1024) void k_lps_force_lds(PD_LPSF_LDS_PARAMS)
+{
+    lps_force_lds_body<DIM, G, GC, false>(PD_LPSF_LDS_ARGS);
+}
+
+template <int DIM, int G, bool GC>
+__global__ __launch_bounds__(1024) void k_lps_force_verlet_lds(PD_LPSF_LDS_PARAMS)
+{
+    lps_force_lds_body<DIM, G, GC, true>(PD_LPSF_LDS_ARGS);
+}
+
+template <int DIM, int G, bool GC, bool FUSE>
+struct ForceKernels {
+    static constexpr auto pmb = FUSE ? k_pmb_verlet_lds<DIM, G, GC> : k_pmb_lds<DIM, G, GC>;
+    static constexpr auto lpsf = FUSE ? k_lps_force_verlet_lds<DIM, G, GC> : k_lps_force_lds<DIM, G, GC>;
+};
 
 inline unsigned tiles_grid(int ntiles) { return (unsigned)((ntiles + 7) / 8 * 8); }
 
@@ -358,6 +432,35 @@ hipError_t lds_attr(K kernel, size_t bytes)
     default: return hipErrorInvalidValue;                 \
     }
 
+// force kernels: also the Verlet epilogue (vf != nullptr) or not
+#define PD_FSWITCH(FN, DIM, G, GC, GCV, VF, ...)                                        \
+    if (VF) {                                                                           \
+        switch (G) {                                                                    \
+        case 4: return FN<DIM, 4, GC, true>(__VA_ARGS__, GCV, *(VF));                   \
+        case 8: return FN<DIM, 8, GC, true>(__VA_ARGS__, GCV, *(VF));                   \
+        case 16: return FN<DIM, 16, GC, true>(__VA_ARGS__, GCV, *(VF));                 \
+        case 32: return FN<DIM, 32, GC, true>(__VA_ARGS__, GCV, *(VF));                 \
+        case 64: return FN<DIM, 64, GC, true>(__VA_ARGS__, GCV, *(VF));                 \
+        default: return hipErrorInvalidValue;                                           \
+        }                                                                               \
+    } else {                                                                            \
+        switch (G) {                                                                    \
+        case 4: return FN<DIM, 4, GC, false>(__VA_ARGS__, GCV, VerletFuse{});           \
+        case 8: return FN<DIM, 8, GC, false>(__VA_ARGS__, GCV, VerletFuse{});           \
+        case 16: return FN<DIM, 16, GC, false>(__VA_ARGS__, GCV, VerletFuse{});         \
+        case 32: return FN<DIM, 32, GC, false>(__VA_ARGS__, GCV, VerletFuse{});         \
+        case 64: return FN<DIM, 64, GC, false>(__VA_ARGS__, GCV, VerletFuse{});         \
+        default: return hipErrorInvalidValue;                                           \
+        }                                                                               \
+    }
+
+#define PD_DISPATCH_FORCE(FN, DIM, G, GCP, VF, ...)                          \
+    if (GCP) {                                                               \
+        PD_FSWITCH(FN, DIM, G, true, *(GCP), VF, __VA_ARGS__)                \
+    } else {                                                                 \
+        PD_FSWITCH(FN, DIM, G, false, GeomCoef{}, VF, __VA_ARGS__)           \
+    }
+
 // gc != nullptr: recompute the bond coefficients from |xi| (no coefficient stream)
 #define PD_DISPATCH_LDS(FN, DIM, G, GCP, ...)                                \
     if (GCP) {                                                               \
@@ -366,17 +469,18 @@ hipError_t lds_attr(K kernel, size_t bytes)
         PD_LSWITCH(FN, DIM, G, false, GeomCoef{}, __VA_ARGS__)               \
     }
 
-template <int DIM, int G, bool GC>
+template <int DIM, int G, bool GC, bool FUSE>
 hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
                      const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window,
-                     double ymin, GeomCoef gc)
+                     double ymin, GeomCoef gc, VerletFuse vf)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
-    hipError_t e = lds_attr(k_pmb_lds<DIM, G, GC>, bytes);
+    constexpr auto kern = ForceKernels<DIM, G, GC, FUSE>::pmb;
+    hipError_t e = lds_attr(kern, bytes);
     if (e != hipSuccess) return e;
-    k_pmb_lds<DIM, G, GC><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order, T.stage_ptr,
-                                                                    T.stage_idx, rp, T.lidx, bnd, fail, rec, f, s0,
-                                                                    (float)s0, stretch_window(s0, window), ymin, gc);
+    kern<<<tiles_grid(T.ntiles), T.block, bytes, s>>>(
+        stop, T.ntiles, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, f, s0, (float)s0,
+        stretch_window(s0, window), ymin, gc, vf);
     return hipGetLastError();
 }
 
@@ -395,17 +499,18 @@ hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const 
     return hipGetLastError();
 }
 
-template <int DIM, int G, bool GC>
+template <int DIM, int G, bool GC, bool FUSE>
 hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
                       const float2 *bnd, uint32_t *fail, const double *rec, float A, float B, double s0,
-                      double window, double ymin, F3 f, GeomCoef gc)
+                      double window, double ymin, F3 f, GeomCoef gc, VerletFuse vf)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
-    hipError_t e = lds_attr(k_lps_force_lds<DIM, G, GC>, bytes);
+    constexpr auto kern = ForceKernels<DIM, G, GC, FUSE>::lpsf;
+    hipError_t e = lds_attr(kern, bytes);
     if (e != hipSuccess) return e;
-    k_lps_force_lds<DIM, G, GC><<<tiles_grid(T.ntiles_int), T.block2, bytes, s>>>(
+    kern<<<tiles_grid(T.ntiles_int), T.block2, bytes, s>>>(
         stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0,
-        (float)s0, stretch_window(s0, window), ymin, f, gc);
+        (float)s0, stretch_window(s0, window), ymin, f, gc, vf);
     return hipGetLastError();
 }
 
@@ -432,13 +537,13 @@ int lds_record_bytes() { return 16 * kLdsRec2; }
 
 hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
                           const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
-                          double window, double ymin, const GeomCoef *gc)
+                          double window, double ymin, const GeomCoef *gc, const VerletFuse *vf)
 {
     if (T.ntiles <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(pmb_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_FORCE(pmb_lds_g, 3, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
     } else {
-        PD_DISPATCH_LDS(pmb_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_FORCE(pmb_lds_g, 2, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
     }
 }
 
@@ -460,12 +565,14 @@ hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
                                 const double *rec, float A, float B, double s0, double window, double ymin, F3 f,
-                                const GeomCoef *gc)
+                                const GeomCoef *gc, const VerletFuse *vf)
 {
     if (T.ntiles_int <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(lpsf_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
+        PD_DISPATCH_FORCE(lpsf_lds_g, 3, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin,
+                          f)
     } else {
-        PD_DISPATCH_LDS(lpsf_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin, f)
+        PD_DISPATCH_FORCE(lpsf_lds_g, 2, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin,
+                          f)
     }
 }

@@ -140,6 +140,47 @@ __global__ void k_verlet21(int dim, int N_int, int n_top, int n_bot, double dt, 
         store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
 }
 
+// Particles the fused LPS force pass does not visit (i >= N_int): the strip
+// motion of half 1 (Solve_Plate_Crack_Grow.cu:20-27) and the u half of the
+// next record buffer.
+__global__ void k_verlet21_tail(int dim, int N_int, int N, int n_top, int n_bot, double dt, double load,
+                                const double *__restrict__ rec_cur, double *rec_next)
+{
+    const int i = N_int + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= N) return;
+    const double *r = rec_cur + (size_t)i * 8 + 4;
+    double uy = r[1];
+    if (i < N_int + n_top)
+        uy += load * dt;
+    else if (i < N_int + n_top + n_bot)
+        uy -= load * dt;
+    store_rec_u(rec_next, i, r[0], uy, dim == 3 ? r[2] : 0.0);
+}
+
+// {v, b} records of a fused Verlet run, and the SoA u and v after it.
+__global__ void k_pack_vrec(int dim, int N, CD3 v, CD3 b, double *vrec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double2 *r = reinterpret_cast<double2 *>(vrec + (size_t)i * 8);
+    r[0] = make_double2(v.p[0][i], v.p[1][i]);
+    r[1] = make_double2(dim == 3 ? v.p[2][i] : 0.0, b.p[0][i]);
+    r[2] = make_double2(b.p[1][i], dim == 3 ? b.p[2][i] : 0.0);
+    r[3] = make_double2(0.0, 0.0);
+}
+
+__global__ void k_unpack_uv(int dim, int N, const double *__restrict__ rec, const double *__restrict__ vrec, D3 u,
+                            D3 v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double *r = rec + (size_t)i * 8 + 4, *w = vrec + (size_t)i * 8;
+    for (int d = 0; d < dim; ++d) {
+        u.p[d][i] = r[d];
+        v.p[d][i] = w[d];
+    }
+}
+
 __global__ void k_verlet2(int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f, CF3 f32, int use32,
                           CD3 b)
 {
@@ -368,6 +409,26 @@ hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bo
     int n = N_int + n_top + n_bot;
     if (n < 1) n = 1;
     k_verlet1<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, load, v, a, u, cyl_state, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_verlet21_tail(hipStream_t s, int dim, int N_int, int N, int n_top, int n_bot, double dt,
+                                double load, const double *rec_cur, double *rec_next)
+{
+    if (N <= N_int) return hipSuccess;
+    k_verlet21_tail<<<nblk(N - N_int, 256), 256, 0, s>>>(dim, N_int, N, n_top, n_bot, dt, load, rec_cur, rec_next);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_vrec(hipStream_t s, int dim, int N, CD3 v, CD3 b, double *vrec)
+{
+    k_pack_vrec<<<nblk(N, 256), 256, 0, s>>>(dim, N, v, b, vrec);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_uv(hipStream_t s, int dim, int N, const double *rec, const double *vrec, D3 u, D3 v)
+{
+    k_unpack_uv<<<nblk(N, 256), 256, 0, s>>>(dim, N, rec, vrec, u, v);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -94,6 +95,8 @@ void free_particles(pd_handle *h)
     dfree(h->minv32);
     dfree(h->theta32);
     dfree(h->rec);
+    dfree(h->rec2);
+    dfree(h->vrec);
     h->rec_u_valid = false;
     dfree(h->order);
     dfree(h->row_ptr);
@@ -161,6 +164,9 @@ int ensure_coef(pd_handle *h)
     if (P.precision == PD_PREC_MIXED) {
         LAUNCH(dalloc(&h->rec, (size_t)h->N * 8));
         LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec));
+        LAUNCH(dalloc(&h->rec2, (size_t)h->N * 8));
+        LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec2));
+        LAUNCH(launch_pack_u(s, nullptr, P.dim, h->N, cd3(h->u), h->rec2));
         // Tiled processing order.  LDS-staged tiles: 5 dx cubes in 3D (about
         // 900 staged records, 70 KB of LDS: two 512-thread workgroups per
         // CU), 16 dx squares in 2D.  Gathering kernels: 16 dx tiles in 3D (a
@@ -404,6 +410,32 @@ int contact_launch(pd_handle *h)
     const pd_params &P = h->P;
     LAUNCH(launch_contact(h->stream, h->red_blocks, h->red_threads, h->N_int, h->dmg, cd3(h->x), h->u, h->v,
                           h->cyl, P.dt, P.dens, P.vol, h->partials, h->S, rec_or_null(h)));
+    return PD_OK;
+}
+
+// LDS tiles, mixed precision: one step's force passes over the records in
+// cur, each row finishing with half 2 of this step and half 1 of the next;
+// the new u goes to the records in next (all rows: the LPS force pass
+// visits rows < N_int, k_verlet21_tail the rest).
+int fused_lds_step(pd_handle *h, double *cur, double *next)
+{
+    const pd_params &P = h->P;
+    hipStream_t s = h->stream;
+    const double ymin = P.dim == 3 ? P.break_y_min : -INFINITY;
+    const VerletFuse vf{h->N_int, P.n_top, P.n_bot, P.dt, P.dens, P.strip_velocity, h->vrec, next};
+    if (P.model == PD_MODEL_PMB) {
+        LAUNCH(launch_pmb_lds(s, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, h->fail, cur,
+                              h->f32, P.s0, P.near_window, ymin, gcp(h), &vf));
+    } else {
+        float ctheta, A, B;
+        lps_mixed_consts(P, &ctheta, &A, &B);
+        LAUNCH(launch_lps_theta_lds(s, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, cur,
+                                    h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
+        LAUNCH(launch_lps_force_lds(s, nullptr, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr, h->bnd,
+                                    h->fail, cur, A, B, P.s0, P.near_window, ymin, h->f32, gcp(h), &vf));
+        LAUNCH(launch_verlet21_tail(s, P.dim, h->N_int, h->N, P.n_top, P.n_bot, P.dt, P.strip_velocity, cur,
+                                    next));
+    }
     return PD_OK;
 }
 
@@ -1067,6 +1099,17 @@ int pd_set_lds(pd_handle *h, int mode)
     return PD_OK;
 }
 
+int pd_set_fused_verlet(pd_handle *h, int mode)
+{
+    // Verlet update inside the LDS force pass of pd_run_verlet: 1 on, 0 or
+    // -1 (default) off.  The results are the same either way.
+    if (!h) return PD_E_INVALID;
+    REQUIRE(mode >= -1 && mode <= 1, "mode must be -1, 0 or 1");
+    h->fused_verlet = mode;
+    drop_graph(h);
+    return PD_OK;
+}
+
 int pd_set_fp64_group(pd_handle *h, int group)
 {
     // Lanes per row of the FP64 PMB kernel: 1 (thread per row), 4, 8, 16,
@@ -1179,6 +1222,34 @@ int pd_run_verlet(pd_handle *h, int nsteps)
         rc = capture(h, kind, 1, [&]() { return step_launch(h); });
         if (rc) return rc;
         for (int s = 0; s < nsteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+    } else if (h->P.precision == PD_PREC_MIXED && h->lds_ok && h->fused_verlet == 1) {
+        // half 1; then steps whose force passes also apply half 2 and the
+        // next half 1, alternating between the two record buffers (a graph
+        // of two steps returns to the first buffer); the last step's force
+        // and half 2.
+        rc = verlet1_launch(h);
+        if (rc) return rc;
+        LAUNCH(dalloc(&h->vrec, (size_t)h->N * 8));
+        LAUNCH(launch_pack_vrec(h->stream, h->P.dim, h->N, cd3(h->v), cd3(h->b), h->vrec));
+        const int fused = nsteps - 1;
+        if (fused >= 2) {
+            rc = capture(h, 5, 1, [&]() {
+                int r = fused_lds_step(h, h->rec, h->rec2);
+                return r ? r : fused_lds_step(h, h->rec2, h->rec);
+            });
+            if (rc) return rc;
+            for (int s = 0; s < fused / 2; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+        }
+        if (fused & 1) {
+            rc = fused_lds_step(h, h->rec, h->rec2);
+            if (rc) return rc;
+            std::swap(h->rec, h->rec2);
+            drop_graph(h);  // captured graphs hold the record pointers
+        }
+        LAUNCH(launch_unpack_uv(h->stream, h->P.dim, h->N, h->rec, h->vrec, h->u, h->v));
+        rc = force_launch(h, nullptr);
+        if (!rc) rc = verlet2_launch(h);
+        if (rc) return rc;
     } else {
         // half 1, then (force, half 2 + half 1 of the next step) from a
         // graph, and the last step's force and half 2

@@ -34,6 +34,20 @@ struct DevScalars {
 
 // Bond coefficients recomputed from |xi| in the mixed kernels (see geom_coef
 // in mixed_bond.h): coef = scale * fac(|xi|) [* 2^(k2 |xi|^2)].
+// Verlet half 2 of the current step and half 1 of the next, applied by the
+// LDS force kernels to each row as soon as its force is final (the
+// arithmetic of k_verlet2 followed by k_verlet1).  During a run v and b live
+// in a 64-byte record per particle, vrec = {v0, v1, v2, b0, b1, b2, 0, 0},
+// u is taken from the row's own particle record, and the new u goes to the
+// other record buffer, rec_next, so tiles still staging the current records
+// never see it; the SoA u and v are unpacked once the run ends.
+struct VerletFuse {
+    int N_int, n_top, n_bot;
+    double dt, dens, load;
+    double *vrec;
+    double *rec_next;
+};
+
 struct GeomCoef {
     float lo, hi, inv_dx;  // fac = 1 up to lo, (hi - |xi|)/dx up to hi, 0 beyond
     float scale;           // vol (LPS omega = w fac vol) or vol*bc (PMB kappa, scr = 1)
@@ -69,9 +83,12 @@ struct pd_handle {
     float *minv32 = nullptr, *theta32 = nullptr;  // mixed LPS: 1/m and dilatation
     double *rec = nullptr;        // mixed mode: 64-byte particle records (x0 | theta/m, 1/m | u)
     bool rec_u_valid = false;     // the u half of every record equals u
+    double *rec2 = nullptr;       // second record buffer of the fused LDS Verlet steps (x0 packed)
+    double *vrec = nullptr;       // {v, b} per particle during fused LDS Verlet runs
     int32_t *order = nullptr;     // mixed mode: tiled processing order (null = index order)
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
     int lds_mode = -1;            // LDS-staged tiles: -1 where they fit (default), 0 off, 1 required
+    int fused_verlet = -1;        // Verlet update inside the LDS force pass: 1 on, 0 or -1 (default) off
     bool lds_ok = false;          // the LDS tile lists are built and in use
     bool lidx_fail_ok = false;    // bit 15 of every lidx word equals the bond's fail bit
     int lds_block = 512;          // threads per workgroup of the LDS kernels (pass 0)
@@ -200,13 +217,14 @@ hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_
                               int32_t *stage_idx, uint16_t *lidx);
 hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
                           const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
-                          double window, double ymin, const GeomCoef *gc);
+                          double window, double ymin, const GeomCoef *gc, const VerletFuse *vf = nullptr);
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, double *rec,
                                 const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
-                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f, const GeomCoef *gc);
+                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f,
+                                const GeomCoef *gc, const VerletFuse *vf = nullptr);
 hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, double *rec);
 hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
                             const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
@@ -221,6 +239,10 @@ hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t 
 // k_integrate.hip
 hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
                           D3 v, CD3 a, D3 u, double *cyl_state, double *rec);
+hipError_t launch_verlet21_tail(hipStream_t s, int dim, int N_int, int N, int n_top, int n_bot, double dt,
+                                double load, const double *rec_cur, double *rec_next);
+hipError_t launch_pack_vrec(hipStream_t s, int dim, int N, CD3 v, CD3 b, double *vrec);
+hipError_t launch_unpack_uv(hipStream_t s, int dim, int N, const double *rec, const double *vrec, D3 u, D3 v);
 hipError_t launch_verlet21(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double dens,
                            double load, D3 v, CD3 f, CF3 f32, int use32, CD3 b, D3 u, double *rec);
 hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f,
