@@ -401,9 +401,6 @@ def main():
         "coefficients_recomputed": recomputed,
         "theta_pass": {"achieved": round(gbs_theta, 1), "frac": round(gbs_theta / HBM_PEAK_GBS, 4),
                        "bytes_per_launch": b_theta, "ms_per_launch": ms_theta},
-        # the timed steps run this force pass with each row's Verlet update
-        # folded in (k_lps_force_verlet_lds); the pass itself is timed alone
-        "step_kernel": "k_lps_force_verlet_lds<3,%d>" % groups[1] if lds_active else kname % groups[1],
     }
     out = {
         "metric": "bond-updates/s & %HBM peak on 1 MI355X; ADR wall time to 1e-6 resid vs plain",
