@@ -383,8 +383,12 @@ __device__ __forceinline__ void lps_force_lds_body(const int *__restrict__ stop,
 #define PD_LPSF_LDS_ARGS \
     stop, ntiles, tile_start, order, stage_ptr, stage_idx, rp, lidx, bnd, fail, rec, A, B, s0, s0f, win, ymin, f, gc, vf
 
+// amdgpu_waves_per_eu(6): at most 80 VGPRs, so that three 512-thread
+// workgroups with 4 dx tiles (about 49 KB of staged records each) share a CU;
+// measured 1.10 ms instead of 1.26 ms per LPS force pass on the 4M-particle
+// block (the tuner then picks 8 lanes per row).
 template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) void k_pmb_lds(PD_PMB_LDS_PARAMS)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1))) void k_pmb_lds(PD_PMB_LDS_PARAMS)
 {
     pmb_lds_body<DIM, G, GC, false>(PD_PMB_LDS_ARGS);
 }
@@ -396,7 +400,7 @@ __global__ __launch_bounds__(1024) void k_pmb_verlet_lds(PD_PMB_LDS_PARAMS)
 }
 
 template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) void k_lps_force_lds(PD_LPSF_LDS_PARAMS)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1))) void k_lps_force_lds(PD_LPSF_LDS_PARAMS)
 {
     lps_force_lds_body<DIM, G, GC, false>(PD_LPSF_LDS_ARGS);
 }
