@@ -229,6 +229,52 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
     return out
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def mpnn_section(n=300, reps=20):
+    """The warm-start network on the graph of one n x n plate: forward and
+    training-step time, the dense (MFMA) flop rate against the BF16 peak,
+    and the per-edge gathers of the message aggregation against HBM."""
+    import numpy as np
+    from pd_gpu1_amd import warmstart as W
+    from pd_gpu1_amd.mpnn import E_FEAT, N_FEAT, OUT, Mpnn
+
+    plate = W.Plate(n, 0.3 * n * W.DX, 0.2)
+    rp, col = plate.rp, plate.col
+    plate.close()
+    N, E = len(rp) - 1, len(col)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((N, N_FEAT)).astype(np.float32)
+    ef = rng.standard_normal((E, E_FEAT)).astype(np.float32)
+    tgt = rng.standard_normal((N, OUT)).astype(np.float32)
+    hid, layers = 64, 3
+    dense = 2.0 * N * (N_FEAT * hid + layers * (hid * hid + 2 * hid * hid) + hid * OUT)
+    gathers = layers * E * (4 * hid + 4 * E_FEAT + 4)  # P row, edge features, col per edge and layer
+    with Mpnn() as net:
+        net.set_graph(rp, col)
+        net.set_features(x, ef, tgt)
+        net.forward()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net.forward()
+        t_fwd = (time.perf_counter() - t0) / reps
+        net.train_step(1e-4)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net.train_step(1e-4, want_loss=False)
+        net.forward()
+        t_train = (time.perf_counter() - t0 - t_fwd) / reps
+    return {"workload": f"warm-start network (3 message-passing layers, hidden 64) on the graph of one {n} x {n} "
+                        f"plate: {N} nodes, {E} directed edges; forward includes the copy of the output to the host",
+            "forward_ms": t_fwd * 1e3, "train_step_ms": t_train * 1e3,
+            "mfma": {"dense_flops_per_forward": dense, "achieved_tflops": dense / t_fwd / 1e12,
+                     "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                     "frac": dense / t_fwd / 1e12 / MFMA_BF16_PEAK_TFLOPS},
+            "aggregation_gathers": {"bytes_per_forward": gathers, "achieved_gbs": gathers / t_fwd / 1e9,
+                                    "frac_of_hbm": gathers / t_fwd / 1e9 / HBM_PEAK_GBS}}
+
+
 def pmb_section(steps=50, warmup=5, reps=10):
     """PMB force on the same 4M-particle block geometry (mixed precision)."""
     from pd_gpu1_amd import _lib, autotune, scenarios
@@ -423,7 +469,10 @@ def main():
                    "launch_shape": {"lds_tiles": lds_active, "tiles": lds_tiles, "max_staged_records": lds_staged,
                                     "tile_cells": shape["tile"], "block": shape["block"]},
                    "autotune_ms": shape.get("ms", {}),
-                   "broken_bonds_after_run": broken},
+                   "broken_bonds_after_run": broken,
+                   # the kernels visit broken bonds too; counting only the bonds
+                   # still intact at the end gives a lower bound of the rate
+                   "intact_bond_updates_per_s_lower_bound": (nb - broken) * args.steps / wall},
         "roofline": roofline,
     }
     if not args.no_pmb:
@@ -436,6 +485,10 @@ def main():
         print(f"[bench] C0 and Kalthoff-Winkler runs {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_adr:
         t = time.perf_counter()
+        try:
+            out["mpnn"] = mpnn_section()
+        except Exception as e:  # reported, never fatal for the main line
+            out["mpnn"] = {"error": str(e)}
         with_cpu = not args.no_cpu_baseline
         out["adr"] = (adr_section(n_batch=512, batch_increments=50, log=True, max_iters=6000, with_cpu=with_cpu)
                       if args.c3_full else adr_section(with_cpu=with_cpu))
