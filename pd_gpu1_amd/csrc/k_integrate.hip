@@ -211,9 +211,50 @@ __device__ __forceinline__ double adr_c(double cn1, double cn2)
     return cn;
 }
 
-__global__ __launch_bounds__(1024) void k_adr_reduce(int dim, int N_int, int N, double mass, double dt, CD3 u,
-                                                     CD3 f, CF3 f32, int use32, CD3 fo, CD3 vho, CD3 b,
-                                                     double *partials, DevScalars *S, double tol)
+// Per-particle terms of the ADR reduction, one thread per particle (the
+// whole GPU), into terms[4][N]: the damping Rayleigh-quotient terms of
+// kernel_cnn (Integrate.cu:19-31) and the residual terms.  The reduction
+// below adds them in the order of the fixed T-lane tree.
+__global__ void k_adr_terms(int dim, int N_int, int N, double mass, double dt, CD3 u, CD3 f, CF3 f32, int use32,
+                            CD3 fo, CD3 vho, CD3 b, const DevScalars *S, double *terms)
+{
+    if (S->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double cn = 0.0, disp = 0.0, rn = 0.0, rd = 0.0;
+    if (i < N_int) {
+        for (int d = 0; d < dim; ++d) {
+            double ud = u.p[d][i];
+            double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
+            double vd = vho.p[d][i];
+            if (d == 0) {
+                if (vd != 0.0) cn = -1.0 * ud * ud * (fd / mass - fo.p[d][i] / mass) / (dt * vd);
+                disp = ud * ud;
+            } else {
+                if (vd != 0.0) cn -= 1.0 * ud * ud * (fd / mass - fo.p[d][i] / mass) / (dt * vd);
+                disp += ud * ud;
+            }
+            double r = fd + b.p[d][i];
+            rn += r * r;
+            rd += b.p[d][i] * b.p[d][i];
+        }
+    } else {
+        for (int d = 0; d < dim; ++d) {
+            double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
+            rd += fd * fd;
+        }
+    }
+    terms[i] = cn;
+    terms[(size_t)N + i] = disp;
+    terms[2 * (size_t)N + i] = rn;
+    terms[3 * (size_t)N + i] = rd;
+}
+
+// Lane t of the T = gridDim.x * blockDim.x lanes adds terms t, t+T, ... in
+// order, then the fixed tree over the lanes (grid_canon); the last block
+// derives c, the residual and the stop flag on the device.
+__global__ __launch_bounds__(1024) void k_adr_reduce(int N, const double *__restrict__ terms, double *partials,
+                                                     DevScalars *S, double tol)
 {
     __shared__ double sh[4][16];
     if (S->done) return;
@@ -221,34 +262,10 @@ __global__ __launch_bounds__(1024) void k_adr_reduce(int dim, int N_int, int N, 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     Vec<4> p{{0.0, 0.0, 0.0, 0.0}};
     for (int i = t; i < N; i += T) {
-        double cn = 0.0, disp = 0.0, rn = 0.0, rd = 0.0;
-        if (i < N_int) {
-            // kernel_cnn, Integrate.cu:19-31
-            for (int d = 0; d < dim; ++d) {
-                double ud = u.p[d][i];
-                double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
-                double vd = vho.p[d][i];
-                if (d == 0) {
-                    if (vd != 0.0) cn = -1.0 * ud * ud * (fd / mass - fo.p[d][i] / mass) / (dt * vd);
-                    disp = ud * ud;
-                } else {
-                    if (vd != 0.0) cn -= 1.0 * ud * ud * (fd / mass - fo.p[d][i] / mass) / (dt * vd);
-                    disp += ud * ud;
-                }
-                double r = fd + b.p[d][i];
-                rn += r * r;
-                rd += b.p[d][i] * b.p[d][i];
-            }
-        } else {
-            for (int d = 0; d < dim; ++d) {
-                double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
-                rd += fd * fd;
-            }
-        }
-        p.v[0] += cn;
-        p.v[1] += disp;
-        p.v[2] += rn;
-        p.v[3] += rd;
+        p.v[0] += terms[i];
+        p.v[1] += terms[(size_t)N + i];
+        p.v[2] += terms[2 * (size_t)N + i];
+        p.v[3] += terms[3 * (size_t)N + i];
     }
     Vec<4> tot;
     if (grid_canon<4>(p, partials, S, sh, tot)) {
@@ -299,63 +316,76 @@ __global__ void k_set_ct(DevScalars *S, int ct, int clear_done)
     }
 }
 
-__global__ __launch_bounds__(1024) void k_contact(int N, const double *__restrict__ dmg, CD3 x, D3 u, D3 v,
-                                                  pd_cyl cyl, double dt, double dens, double vol,
-                                                  double *partials, DevScalars *S, double *rec)
+// Contact of every particle with the cylinder, one thread per particle:
+// Solve_Kalthoff_Winkler.cu:177-239 moves the particles inside the cylinder
+// to its surface and writes their reaction terms to terms[3][N] (zero for the
+// others).  The reference sums the reaction by float atomicAdd
+// (Solve_Kalthoff_Winkler.cu:168-170); k_contact_sum adds the terms in the
+// order of the fixed T-lane tree.
+__global__ void k_contact_terms(int N, const double *__restrict__ dmg, CD3 x, D3 u, D3 v, pd_cyl cyl, double dt,
+                                double dens, double vol, const DevScalars *S, double *terms, double *rec)
 {
-    // Solve_Kalthoff_Winkler.cu:177-239; the reaction is summed in canonical
-    // order instead of by float atomicAdd (Solve_Kalthoff_Winkler.cu:168-170),
-    // then the cylinder's second half step (Solve_Kalthoff_Winkler.cu:255-265)
-    // runs in the last block.
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double *cd = S->cyl + 3;
+    const double cdx = cd[0], cdy = cd[1], cdz = cd[2];
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    if (dmg[i] < 1.0) {
+        double dpx = x.p[0][i] + u.p[0][i] - cyl.x - cdx;
+        double dpy = x.p[1][i] + u.p[1][i] - cyl.y - cdy;
+        double dpz = x.p[2][i] + u.p[2][i] - cyl.z - cdz;
+        double dpd = sqrt(dpx * dpx + dpz * dpz);
+        if (dpd <= cyl.rad && fabs(dpy) <= (cyl.height / 2.0)) {
+            double rx, ry, rz;
+            if (cyl.rad - dpd <= cyl.height / 2.0 - fabs(dpy)) {
+                rx = dpx / dpd * cyl.rad;
+                ry = dpy;
+                rz = dpz / dpd * cyl.rad;
+            } else if (dpy >= 0.0) {
+                rx = dpx;
+                ry = cyl.height / 2.0;
+                rz = dpz;
+            } else {
+                rx = dpx;
+                ry = -cyl.height / 2.0;
+                rz = dpz;
+            }
+            double oux = u.p[0][i], ouy = u.p[1][i], ouz = u.p[2][i];
+            double ovx = v.p[0][i], ovy = v.p[1][i], ovz = v.p[2][i];
+            double nux = cyl.x + cdx + rx - x.p[0][i];
+            double nuy = cyl.y + cdy + ry - x.p[1][i];
+            double nuz = cyl.z + cdz + rz - x.p[2][i];
+            u.p[0][i] = nux;
+            u.p[1][i] = nuy;
+            u.p[2][i] = nuz;
+            if (rec) store_rec_u(rec, i, nux, nuy, nuz);
+            double nvx = (nux - oux) / dt, nvy = (nuy - ouy) / dt, nvz = (nuz - ouz) / dt;
+            v.p[0][i] = nvx;
+            v.p[1][i] = nvy;
+            v.p[2][i] = nvz;
+            r0 = -dens * (nvx - ovx) / dt * vol;
+            r1 = -dens * (nvy - ovy) / dt * vol;
+            r2 = -dens * (nvz - ovz) / dt * vol;
+        }
+    }
+    terms[i] = r0;
+    terms[(size_t)N + i] = r1;
+    terms[2 * (size_t)N + i] = r2;
+}
+
+// The reaction on the cylinder in canonical order, then the cylinder's second
+// half step (Solve_Kalthoff_Winkler.cu:255-265) in the last block.
+__global__ __launch_bounds__(1024) void k_contact_sum(int N, const double *__restrict__ terms, pd_cyl cyl,
+                                                      double dt, double *partials, DevScalars *S)
+{
     __shared__ double sh[3][16];
     const int T = gridDim.x * blockDim.x;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const double *cd = S->cyl + 3;
-    const double cdx = cd[0], cdy = cd[1], cdz = cd[2];
     Vec<3> p{{0.0, 0.0, 0.0}};
     for (int i = t; i < N; i += T) {
-        double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-        if (dmg[i] < 1.0) {
-            double dpx = x.p[0][i] + u.p[0][i] - cyl.x - cdx;
-            double dpy = x.p[1][i] + u.p[1][i] - cyl.y - cdy;
-            double dpz = x.p[2][i] + u.p[2][i] - cyl.z - cdz;
-            double dpd = sqrt(dpx * dpx + dpz * dpz);
-            if (dpd <= cyl.rad && fabs(dpy) <= (cyl.height / 2.0)) {
-                double rx, ry, rz;
-                if (cyl.rad - dpd <= cyl.height / 2.0 - fabs(dpy)) {
-                    rx = dpx / dpd * cyl.rad;
-                    ry = dpy;
-                    rz = dpz / dpd * cyl.rad;
-                } else if (dpy >= 0.0) {
-                    rx = dpx;
-                    ry = cyl.height / 2.0;
-                    rz = dpz;
-                } else {
-                    rx = dpx;
-                    ry = -cyl.height / 2.0;
-                    rz = dpz;
-                }
-                double oux = u.p[0][i], ouy = u.p[1][i], ouz = u.p[2][i];
-                double ovx = v.p[0][i], ovy = v.p[1][i], ovz = v.p[2][i];
-                double nux = cyl.x + cdx + rx - x.p[0][i];
-                double nuy = cyl.y + cdy + ry - x.p[1][i];
-                double nuz = cyl.z + cdz + rz - x.p[2][i];
-                u.p[0][i] = nux;
-                u.p[1][i] = nuy;
-                u.p[2][i] = nuz;
-                if (rec) store_rec_u(rec, i, nux, nuy, nuz);
-                double nvx = (nux - oux) / dt, nvy = (nuy - ouy) / dt, nvz = (nuz - ouz) / dt;
-                v.p[0][i] = nvx;
-                v.p[1][i] = nvy;
-                v.p[2][i] = nvz;
-                r0 = -dens * (nvx - ovx) / dt * vol;
-                r1 = -dens * (nvy - ovy) / dt * vol;
-                r2 = -dens * (nvz - ovz) / dt * vol;
-            }
-        }
-        p.v[0] += r0;
-        p.v[1] += r1;
-        p.v[2] += r2;
+        p.v[0] += terms[i];
+        p.v[1] += terms[(size_t)N + i];
+        p.v[2] += terms[2 * (size_t)N + i];
     }
     Vec<3> tot;
     if (grid_canon<3>(p, partials, S, sh, tot)) {
@@ -451,10 +481,12 @@ hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double d
 
 hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, int N_int, int N, double mass,
                              double dt, CD3 u, CD3 f, CF3 f32, int use32, CD3 fo, CD3 vho, CD3 b,
-                             double *partials, DevScalars *S, double tol)
+                             double *partials, DevScalars *S, double tol, double *terms)
 {
-    k_adr_reduce<<<blocks, threads, 0, s>>>(dim, N_int, N, mass, dt, u, f, f32, use32, fo, vho, b, partials, S,
-                                            tol);
+    k_adr_terms<<<nblk(N, 256), 256, 0, s>>>(dim, N_int, N, mass, dt, u, f, f32, use32, fo, vho, b, S, terms);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_adr_reduce<<<blocks, threads, 0, s>>>(N, terms, partials, S, tol);
     return hipGetLastError();
 }
 
@@ -475,9 +507,12 @@ hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done)
 
 hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u, D3 v,
                           pd_cyl cyl, double dt, double dens, double vol, double *partials, DevScalars *S,
-                          double *rec)
+                          double *rec, double *terms)
 {
-    k_contact<<<blocks, threads, 0, s>>>(N, dmg, x, u, v, cyl, dt, dens, vol, partials, S, rec);
+    k_contact_terms<<<nblk(N, 256), 256, 0, s>>>(N, dmg, x, u, v, cyl, dt, dens, vol, S, terms, rec);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_contact_sum<<<blocks, threads, 0, s>>>(N, terms, cyl, dt, partials, S);
     return hipGetLastError();
 }
 

@@ -97,6 +97,7 @@ void free_particles(pd_handle *h)
     dfree(h->rec);
     dfree(h->rec2);
     dfree(h->vrec);
+    dfree(h->terms);
     h->rec_u_valid = false;
     dfree(h->order);
     dfree(h->row_ptr);
@@ -409,7 +410,7 @@ int contact_launch(pd_handle *h)
 {
     const pd_params &P = h->P;
     LAUNCH(launch_contact(h->stream, h->red_blocks, h->red_threads, h->N_int, h->dmg, cd3(h->x), h->u, h->v,
-                          h->cyl, P.dt, P.dens, P.vol, h->partials, h->S, rec_or_null(h)));
+                          h->cyl, P.dt, P.dens, P.vol, h->partials, h->S, rec_or_null(h), h->terms));
     return PD_OK;
 }
 
@@ -466,7 +467,7 @@ int adr_iter_launch(pd_handle *h, double tol)
     if (rc) return rc;
     LAUNCH(launch_adr_reduce(h->stream, h->red_blocks, h->red_threads, P.dim, h->N_int, h->N, P.mass, P.dt,
                              cd3(h->u), cd3(h->f), cf3(h->f32), use32(h), cd3(h->fo), cd3(h->vho), cd3(h->b),
-                             h->partials, h->S, tol));
+                             h->partials, h->S, tol, h->terms));
     LAUNCH(launch_adr_update(h->stream, P.dim, h->N_int, P.mass, P.dt, cd3(h->f), cf3(h->f32), use32(h),
                              cd3(h->b), h->vh, h->v, h->vho, h->u, h->fo, h->S, rec_or_null(h)));
     return PD_OK;
@@ -768,6 +769,7 @@ int pd_set_positions(pd_handle *h, const double *x, const double *y, const doubl
     }
     LAUNCH(dalloc(&h->dmg, n));
     LAUNCH(hipMemset(h->dmg, 0, sizeof(double) * n));
+    LAUNCH(dalloc(&h->terms, 4 * (size_t)n));
     LAUNCH(dalloc(&h->m, n));
     LAUNCH(dalloc(&h->theta, n));
     LAUNCH(hipMemset(h->theta, 0, sizeof(double) * n));

@@ -85,6 +85,7 @@ struct pd_handle {
     bool rec_u_valid = false;     // the u half of every record equals u
     double *rec2 = nullptr;       // second record buffer of the fused LDS Verlet steps (x0 packed)
     double *vrec = nullptr;       // {v, b} per particle during fused LDS Verlet runs
+    double *terms = nullptr;      // [4][N] per-particle terms of the ADR and contact reductions
     int32_t *order = nullptr;     // mixed mode: tiled processing order (null = index order)
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
     int lds_mode = -1;            // LDS-staged tiles: -1 where they fit (default), 0 off, 1 required
@@ -249,13 +250,13 @@ hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double d
                           CF3 f32, int use32, CD3 b);
 hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, int N_int, int N,
                              double mass, double dt, CD3 u, CD3 f, CF3 f32, int use32, CD3 fo, CD3 vho,
-                             CD3 b, double *partials, DevScalars *S, double tol);
+                             CD3 b, double *partials, DevScalars *S, double tol, double *terms);
 hipError_t launch_adr_update(hipStream_t s, int dim, int N_int, double mass, double dt, CD3 f,
                              CF3 f32, int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo,
                              DevScalars *S, double *rec);
 hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u,
                           D3 v, pd_cyl cyl, double dt, double dens, double vol, double *partials,
-                          DevScalars *S, double *rec);
+                          DevScalars *S, double *rec, double *terms);
 hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done);
 hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double *v, int64_t n,
                             double *partials, DevScalars *S);
