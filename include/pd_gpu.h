@@ -289,7 +289,7 @@ int pd_set_lds(pd_handle *h, int mode);
  * separate particle passes.  Same results either way. */
 int pd_set_fused_verlet(pd_handle *h, int mode);
 /* Lanes per row of the FP64 PMB kernel: 1 (thread per row), 4, 8, 16, 32, or
- * -1 = 4 in 2D and 16 in 3D (default).  Terms are added in bond order, so the
+ * -1 = 8 in 2D and 16 in 3D (default).  Terms are added in bond order, so the
  * forces and breaks are identical for every choice. */
 int pd_set_fp64_group(pd_handle *h, int group);
 /* Bond coefficients of the mixed kernels (|xi| and kappa or omega = w fac vol):

@@ -261,7 +261,20 @@ __global__ __launch_bounds__(1024) void k_adr_reduce(int N, const double *__rest
     const int T = gridDim.x * blockDim.x;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     Vec<4> p{{0.0, 0.0, 0.0, 0.0}};
-    for (int i = t; i < N; i += T) {
+    int i = t;
+    // four strided groups of loads in flight, added in the same order
+    for (; i + 3 * T < N; i += 4 * T) {
+        double q[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) q[u][c] = terms[(size_t)c * N + i + u * T];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) p.v[c] += q[u][c];
+    }
+    for (; i < N; i += T) {
         p.v[0] += terms[i];
         p.v[1] += terms[(size_t)N + i];
         p.v[2] += terms[2 * (size_t)N + i];
@@ -382,7 +395,19 @@ __global__ __launch_bounds__(1024) void k_contact_sum(int N, const double *__res
     const int T = gridDim.x * blockDim.x;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     Vec<3> p{{0.0, 0.0, 0.0}};
-    for (int i = t; i < N; i += T) {
+    int i = t;
+    for (; i + 3 * T < N; i += 4 * T) {
+        double q[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[u][c] = terms[(size_t)c * N + i + u * T];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p.v[c] += q[u][c];
+    }
+    for (; i < N; i += T) {
         p.v[0] += terms[i];
         p.v[1] += terms[(size_t)N + i];
         p.v[2] += terms[2 * (size_t)N + i];
