@@ -381,8 +381,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py: this engine runs one specimen on one GPU; multi-GPU runs are not part of "
-              "this build (see the sharding design).  Use --gpus 1.", file=sys.stderr)
+        print("bench.py: the engine runs one specimen on one GPU; --gpus above 1 is not supported "
+              "(sharding across GPUs exists as a design only).  Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         return

@@ -1,7 +1,7 @@
 """FP64 numpy reference of the message-passing warm-start network (test
 infrastructure only; the GPU network lives in pd_gpu1_amd/csrc/k_mpnn.hip).
 
-Network (no reference counterpart; defined by this build):
+Network (no reference counterpart; the warm-start network of pd_gpu1_amd):
     h0 = silu(x W0 + b0)                                   x: [N, 8]
     for l in 1..3:
         P   = h W1_l                                       [N, 64]

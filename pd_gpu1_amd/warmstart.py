@@ -15,7 +15,7 @@ start also moves the free particles by a predicted increment:
   strips, du_base = (0, du * y / y_strip), plus the correction the
   message-passing network (pd_mpnn_*) predicts from the plate's bond graph:
   du_pred = du_base + du * (0, y_net_y).  Without the network (the baseline
-  the network is judged against) du_base alone is used.  Only the y part of
+  the network is compared with) du_base alone is used.  Only the y part of
   the prediction is applied: measured on unseen plates, any error in the
   x displacement of the guess (even with rigid-body modes removed) leaves a
   slowly damped mode that multiplies the ADR iteration count by 4-20, while
