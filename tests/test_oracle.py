@@ -409,3 +409,81 @@ def test_c0_short_run_deterministic(seed):
         runs.append((m.stack("U"), m.fail.copy()))
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
     assert np.max(np.abs(runs[0][0][1, 10000:10300])) == pytest.approx(20.0 * 1e-8 * (20 + seed), rel=1e-12)
+
+
+# ------------------------------------------------------ remaining functions
+def test_kw_notches_geometry_and_symmetry():
+    # Solve_Kalthoff_Winkler.cu:65-87: a bond breaks when it crosses one of the
+    # notch lines x = -rad or x = +rad and one of its ends lies at y >= 0.
+    dx, rad = 1.0e-3, 0.0063
+    x, y, z = O.coord_kw(41, 21, 3, dx, rad)
+    delta = 3.015 * dx
+    rp, col, _ = O.neighbors_cell(3, x, y, z, delta)
+    fail = np.ones(len(col), dtype=np.uint8)
+    O.set_crack_kw(len(x), rp, col, x, y, rad, fail)
+    rows = np.repeat(np.arange(len(x)), np.diff(rp))
+    xi, xc, yi, yc = x[rows], x[col], y[rows], y[col]
+    cross = ((xi + rad) * (xc + rad) < 0) | ((xi - rad) * (xc - rad) < 0)
+    upper = (yi >= -1e-10) | (yc >= -1e-10)
+    broken = fail == 0
+    assert broken.sum() > 100
+    assert np.all(cross[broken]) and np.all(upper[broken])   # only crossing bonds in the notch region
+    assert np.all(broken[cross & upper])                     # and all of them
+    # the pair i->j and j->i break together
+    key = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(rows, col))}
+    assert all(broken[key[(int(b), int(a))]] == broken[k] for k, (a, b) in enumerate(zip(rows, col)))
+    # mirror symmetry x -> -x of the lattice maps the broken set onto itself
+    pos = {(round(a / dx * 4), round(b / dx * 4), round(c / dx * 4)): i for i, (a, b, c) in enumerate(zip(x, y, z))}
+    mirror = np.array([pos[(round(-a / dx * 4), round(b / dx * 4), round(c / dx * 4))] for a, b, c in zip(x, y, z)])
+    mb = {(int(mirror[a]), int(mirror[b])) for a, b in zip(rows[broken], col[broken])}
+    assert mb == {(int(a), int(b)) for a, b in zip(rows[broken], col[broken])}
+
+
+def test_adr_integrate_relaxes_a_spring_to_its_static_solution():
+    # One free particle on a linear spring, F = -k u, under a body force b:
+    # ADR (Integrate.cu:62-129) must settle at u = b / k.  A flipped sign or a
+    # dropped term in the update or the damping estimate does not.
+    k, mass, dt = 0.1, 1.0, 1.0
+    U, V, VH, VHO, FO = ([np.zeros(1), np.zeros(1)] for _ in range(5))
+    B = [np.array([1.0]), np.array([-2.0])]
+    for it in range(400):
+        F = [-k * U[0], -k * U[1]]
+        cn, disp = O.adr_terms(2, 1, mass, dt, U, F, FO, VHO)
+        c = O.adr_c(float(cn.sum()), float(disp.sum()))
+        if 3 <= it < 60:  # later f - f_old and v_old both vanish and the quotient loses digits
+            assert abs(c - 2.0 * math.sqrt(k / mass)) < 1e-6  # the local Rayleigh quotient of the spring
+        O.adr_integrate(2, 1, it, c, mass, dt, F, B, VH, V, VHO, U, FO)
+    assert abs(U[0][0] - 10.0) < 1e-9 and abs(U[1][0] + 20.0) < 1e-9
+    assert abs(V[0][0]) < 1e-9 and abs(V[1][0]) < 1e-9
+
+
+def test_resid_terms_and_strip_load_hand_made():
+    # free particles: |F + b|^2 and |b|^2; strip particles: |F|^2
+    F = [np.array([3.0, 1.0, 6.0]), np.array([4.0, 0.0, 8.0])]
+    B = [np.array([0.0, 1.0, 9.0]), np.array([0.0, 2.0, 9.0])]
+    rnum, rden = O.resid_terms(2, 2, 3, F, B)
+    assert rnum.tolist() == [25.0, 8.0, 0.0] and rden.tolist() == [0.0, 5.0, 100.0]
+    # Solve_Plate_Crack_Grow.cu:20-27: top strip up, bottom strip down by load*dt
+    uy = np.array([1.0, 2.0, 0.0, 0.0, 0.0, 7.0])
+    O.quick_load(2, 2, 1, 20.0, 1e-3, uy)
+    assert uy.tolist() == [1.0, 2.0, 0.02, 0.02, -0.02, 7.0]
+
+
+def test_cylinder_half_steps_follow_the_constant_force_parabola():
+    # Solve_Kalthoff_Winkler.cu:24-37,101-110,255-265: with a constant force
+    # from the first step on, velocity Verlet starting at rest acceleration
+    # gives disp_n = n v0 dt + a dt^2 n (n - 1) / 2 exactly (in exact arithmetic).
+    rad, mass, height, v0 = 0.025, 1.57, 0.05, -32.0
+    st = O.cylinder_init(np.array([rad, mass, height, v0, 0.0, 0.1, 0.0]))
+    assert st["vel"].tolist() == [0.0, v0, 0.0] and not st["disp"].any()
+    dt, force = 1e-6, np.array([0.3, -2.0e3, 0.05])
+    n = 200
+    for _ in range(n):
+        O.cylinder_half1(dt, st)
+        st["force"][:] = force
+        O.cylinder_half2(dt, mass, st)
+        assert not st["force"].any()  # consumed by the half step
+    a = force / mass
+    expect = np.array([0.0, n * v0 * dt, 0.0]) + a * dt * dt * n * (n - 1) / 2
+    assert np.allclose(st["disp"], expect, rtol=1e-10, atol=1e-15)
+    assert np.allclose(st["vel"], np.array([0.0, v0, 0.0]) + a * dt * (n - 0.5), rtol=1e-10, atol=1e-15)
