@@ -1220,11 +1220,22 @@ int pd_run_verlet(pd_handle *h, int nsteps)
     if (rc) return rc;
     LAUNCH(sync_rec_u(h));
     LAUNCH(sync_lidx_fail(h));
+    // Long runs replay graphs of kGraphSteps steps (one launch per graph
+    // instead of per step); the remainder is launched directly.
+    constexpr int kGraphSteps = 8;
     if (h->have_cyl || nsteps == 1) {
         const int kind = 1 + (h->have_cyl ? 1 : 0);
-        rc = capture(h, kind, 1, [&]() { return step_launch(h); });
-        if (rc) return rc;
-        for (int s = 0; s < nsteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+        if (nsteps >= kGraphSteps) {
+            rc = capture(h, 20 + kind, kGraphSteps, [&]() { return step_launch(h); });
+            if (rc) return rc;
+            for (int s = 0; s < nsteps / kGraphSteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+            for (int s = 0; s < nsteps % kGraphSteps && !rc; ++s) rc = step_launch(h);
+            if (rc) return rc;
+        } else {
+            rc = capture(h, kind, 1, [&]() { return step_launch(h); });
+            if (rc) return rc;
+            for (int s = 0; s < nsteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+        }
     } else if (h->P.precision == PD_PREC_MIXED && h->lds_ok && h->fused_verlet == 1) {
         // half 1; then steps whose force passes also apply half 2 and the
         // next half 1, alternating between the two record buffers (a graph
@@ -1258,9 +1269,18 @@ int pd_run_verlet(pd_handle *h, int nsteps)
         // graph, and the last step's force and half 2
         rc = verlet1_launch(h);
         if (rc) return rc;
-        rc = capture(h, 4, 1, [&]() { return fused_step_launch(h); });
-        if (rc) return rc;
-        for (int s = 0; s < nsteps - 1; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+        const int mid = nsteps - 1;
+        if (mid >= kGraphSteps) {
+            rc = capture(h, 24, kGraphSteps, [&]() { return fused_step_launch(h); });
+            if (rc) return rc;
+            for (int s = 0; s < mid / kGraphSteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+            for (int s = 0; s < mid % kGraphSteps && !rc; ++s) rc = fused_step_launch(h);
+            if (rc) return rc;
+        } else {
+            rc = capture(h, 4, 1, [&]() { return fused_step_launch(h); });
+            if (rc) return rc;
+            for (int s = 0; s < mid; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
+        }
         rc = force_launch(h, nullptr);
         if (!rc) rc = verlet2_launch(h);
         if (rc) return rc;
