@@ -85,3 +85,22 @@ def test_crack_tip_advances_in_the_pulled_plate():
     assert lengths[0] >= 0.005 - 2 * sc.p.dx       # the 10 mm pre-crack
     assert lengths[-1] > lengths[0]                 # the crack has grown
     assert ob.crack_speed(times, lengths) > 0.0
+
+
+def test_kalthoff_winkler_crack_angles():
+    # Both notch cracks of the impact run leave their tips at a mode-II kink
+    # angle (the experiment's is about 70 degrees from the notch) and mirror
+    # each other about the centre line, as the specimen and the impactor do.
+    from pd_gpu1_amd import observables as ob
+    sc = scenarios.c1_kalthoff_winkler()
+    with sc.make(precision=_lib.PD_PREC_MIXED) as prob:
+        angles = []
+        for steps in (900, 450):
+            prob.run_verlet(steps)
+            prob.damage()
+            angles.append(ob.kw_crack_angles(sc.x, sc.y, prob.get("DMG"), 0.025, sc.p.dx))
+    print("KW crack angles (left, right) after 900 and 1350 steps:", angles, flush=True)
+    left, right = angles[-1]
+    assert np.isfinite(left) and np.isfinite(right)
+    assert 45.0 < left < 85.0 and 45.0 < right < 85.0
+    assert abs(left - right) < 2.0

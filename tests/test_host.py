@@ -33,3 +33,27 @@ def test_crack_half_length_and_speed():
     assert ob.crack_half_length(np.zeros(nx * ny), nx, ny, dx) == 0.0
     t = np.array([0.0, 1e-6, 2e-6, 3e-6])
     assert abs(ob.crack_speed(t, 0.004 + 500.0 * t) - 500.0) < 1e-6
+
+
+def test_kw_crack_angles_recover_synthetic_cracks():
+    from pd_gpu1_amd import observables as ob
+    rng = np.random.default_rng(3)
+    rad, dx = 0.025, 1e-3
+    xs, ys, dm = [], [], []
+    for side, ang in ((-1.0, 68.0), (1.0, 72.0)):
+        t = np.linspace(3 * dx, 0.035, 60)
+        a = np.radians(ang)
+        xs.append(side * rad + side * t * np.sin(a) + rng.normal(0, 0.2 * dx, t.size))
+        ys.append(-t * np.cos(a) + rng.normal(0, 0.2 * dx, t.size))
+        dm.append(np.full(t.size, 0.5))
+    # undamaged lattice and damage above the tips (the notch itself) are ignored
+    gx, gy = np.meshgrid(np.linspace(-0.1, 0.1, 60), np.linspace(-0.05, 0.05, 30))
+    xs.append(gx.ravel()); ys.append(gy.ravel()); dm.append(np.zeros(gx.size))
+    xs.append(np.full(20, rad)); ys.append(np.linspace(0.001, 0.04, 20)); dm.append(np.full(20, 0.45))
+    left, right = ob.kw_crack_angles(np.concatenate(xs), np.concatenate(ys), np.concatenate(dm), rad, dx)
+    assert abs(left - 68.0) < 1.5 and abs(right - 72.0) < 1.5
+    # a crack turning towards the centre line has a negative angle
+    t = np.linspace(3 * dx, 0.03, 40)
+    inward = ob.crack_angle_from_tip(rad - t * np.sin(0.5), -t * np.cos(0.5), rad, 0.0, 1.0)
+    assert abs(inward + np.degrees(0.5)) < 1e-6
+    assert np.isnan(ob.crack_angle_from_tip([], [], 0.0, 0.0, 1.0))
