@@ -251,11 +251,10 @@ __device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int n
         double ax = 0.0, ay = 0.0, az = 0.0;
 #define PMB_BODY(R, S, K)                                                                        \
     {                                                                                            \
-        _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<false>(g.xi2, gc, L0, cf, rL0);                                        \
-        const float s = g.num * frcp(L0 * (g.L + L0));                                           \
+        const float s = g.num * frcp(L0 * g.len_sum(L0));                                           \
         const float tb = masked(cf * s * g.rL, S.mask);                                      \
         ax += (double)(tb * g.dx);                                                               \
         ay += (double)(tb * g.dy);                                                               \
@@ -299,11 +298,10 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
         double acc = 0.0;
 #define THETA_BODY(R, S, K)                                                                      \
     {                                                                                            \
-        _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
-        const float e = g.num * frcp(g.L + L0); /* |xi+eta| - |xi| */                            \
+        const float e = g.num * frcp(g.len_sum(L0)); /* |xi+eta| - |xi| */                            \
         acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
         PD_LDS_ROW_WALK(need_intact, THETA_BODY)
@@ -338,13 +336,12 @@ __device__ __forceinline__ void lps_force_lds_body(const int *__restrict__ stop,
         double ax = 0.0, ay = 0.0, az = 0.0;
 #define LPSF_BODY(R, S, K)                                                                       \
     {                                                                                            \
-        _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
-        const float e = g.num * frcp(g.L + L0);                                                  \
+        const float e = g.num * frcp(g.len_sum(L0));                                                  \
         const float s = e * (GC ? rL0 : frcp(L0));                                               \
-        const float tt = masked(cf * (A * L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        const float tt = masked(cf * __builtin_fmaf(A * L0, qi.q + R.q, B * e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \

@@ -67,11 +67,10 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
     double ax = 0.0, ay = 0.0, az = 0.0;
 #define PMB_BODY(R, S, K)                                                                        \
     {                                                                                            \
-        _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<false>(g.xi2, gc, L0, cf, rL0);                                        \
-        const float s = g.num * frcp(L0 * (g.L + L0));                                           \
+        const float s = g.num * frcp(L0 * g.len_sum(L0));                                           \
         const float t = masked(cf * s * g.rL, S.mask);                                       \
         ax += (double)(t * g.dx);                                                                \
         ay += (double)(t * g.dy);                                                                \
@@ -108,11 +107,10 @@ __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__
     const bool need_intact = skip_broken != 0;
 #define THETA_BODY(R, S, K)                                                                      \
     {                                                                                            \
-        _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
-        const float e = g.num * frcp(g.L + L0); /* |xi+eta| - |xi| */                            \
+        const float e = g.num * frcp(g.len_sum(L0)); /* |xi+eta| - |xi| */                            \
         acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
     PD_ROW_WALK(need_intact, THETA_BODY)
@@ -145,13 +143,12 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
     // (t_i + t_j) vol fac = omega [A xi (q_i + q_j) + B e (r_i + r_j)]
 #define LPSF_BODY(R, S, K)                                                                       \
     {                                                                                            \
-        _Pragma("clang fp contract(fast)")                                                       \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
-        const float e = g.num * frcp(g.L + L0);                                                  \
+        const float e = g.num * frcp(g.len_sum(L0));                                                  \
         const float s = e * (GC ? rL0 : frcp(L0));                                               \
-        const float tt = masked(cf * (A * L0 * (qi.q + R.q) + B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        const float tt = masked(cf * __builtin_fmaf(A * L0, qi.q + R.q, B * e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
@@ -195,16 +192,15 @@ __global__ void k_pack_u(const int *__restrict__ stop, int dim, int N, CD3 u, do
 __global__ void k_geom_check(int dim, int N, const int32_t *__restrict__ rp, const int32_t *__restrict__ col, CD3 x,
                              const float2 *__restrict__ bnd, GeomCoef gc, int *ok)
 {
-#pragma clang fp contract(fast)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
         const int c = col[k];
         const float ex = (float)(x.p[0][c] - x.p[0][i]), ey = (float)(x.p[1][c] - x.p[1][i]);
-        float xi2 = ex * ex + ey * ey;
+        float xi2 = __builtin_fmaf(ey, ey, ex * ex);  // as geo() forms it
         if (dim == 3) {
             const float ez = (float)(x.p[2][c] - x.p[2][i]);
-            xi2 += ez * ez;
+            xi2 = __builtin_fmaf(ez, ez, xi2);
         }
         float L0, cf, rL0;
         if (gc.k2 != 0.f)

@@ -87,23 +87,12 @@ __device__ __forceinline__ bool grid_canon(Vec<NC> &p, double *partials, DevScal
     return true;
 }
 
-__global__ void k_verlet1(int dim, int N_int, int n_top, int n_bot, double dt, double load, D3 v, CD3 a,
-                          D3 u, double *cyl, double *rec)
+template <int DIM>
+__global__ void k_verlet1(int N_int, int n_top, int n_bot, double dt, double load, D3 v, CD3 a, D3 u,
+                          double *cyl, double *rec)
 {
     // Integrate.cu:447-463 (i < N_int) and Solve_Plate_Crack_Grow.cu:20-27
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N_int) {
-        for (int d = 0; d < dim; ++d) {
-            v.p[d][i] += (dt / 2.0) * a.p[d][i];
-            u.p[d][i] += v.p[d][i] * dt;
-        }
-    } else if (i < N_int + n_top) {
-        u.p[1][i] += load * dt;
-    } else if (i < N_int + n_top + n_bot) {
-        u.p[1][i] -= load * dt;
-    }
-    if (rec && i < N_int + n_top + n_bot)
-        store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (cyl && i == 0) {
         // Solve_Kalthoff_Winkler.cu:101-110: vel += dt/2*acc; disp += vel*dt
         for (int d = 0; d < 3; ++d) {
@@ -111,6 +100,26 @@ __global__ void k_verlet1(int dim, int N_int, int n_top, int n_bot, double dt, d
             cyl[3 + d] += cyl[d] * dt;
         }
     }
+    if (i >= N_int + n_top + n_bot) return;
+    double ud[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < DIM; ++d) ud[d] = u.p[d][i];
+    if (i < N_int) {
+        double vd[DIM], ad[DIM];
+        for (int d = 0; d < DIM; ++d) {
+            vd[d] = v.p[d][i];
+            ad[d] = a.p[d][i];
+        }
+        for (int d = 0; d < DIM; ++d) {
+            vd[d] += (dt / 2.0) * ad[d];
+            v.p[d][i] = vd[d];
+            ud[d] += vd[d] * dt;
+            u.p[d][i] = ud[d];
+        }
+    } else {
+        ud[1] = i < N_int + n_top ? ud[1] + load * dt : ud[1] - load * dt;
+        u.p[1][i] = ud[1];
+    }
+    if (rec) store_rec_u(rec, i, ud[0], ud[1], ud[2]);
 }
 
 // Half 2 of one step and half 1 of the next in one pass (runs without the
@@ -118,26 +127,35 @@ __global__ void k_verlet1(int dim, int N_int, int n_top, int n_bot, double dt, d
 // by k_verlet1, with the acceleration kept in a register.  The acceleration
 // of a step that another step follows is never read by the API, so only
 // the last step's k_verlet2 stores it.
-__global__ void k_verlet21(int dim, int N_int, int n_top, int n_bot, double dt, double dens, double load, D3 v,
-                           CD3 f, CF3 f32, int use32, CD3 b, D3 u, double *rec)
+template <int DIM, bool F32>
+__global__ void k_verlet21(int N_int, int n_top, int n_bot, double dt, double dens, double load, D3 v, CD3 f,
+                           CF3 f32, CD3 b, D3 u, double *rec)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N_int + n_top + n_bot) return;
+    // every load of the particle is issued before the arithmetic
+    double ud[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < DIM; ++d) ud[d] = u.p[d][i];
     if (i < N_int) {
-        for (int d = 0; d < dim; ++d) {
-            const double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
-            const double ad = (fd + b.p[d][i]) / dens;  // Integrate.cu:465-481
-            double vd = v.p[d][i] + (dt / 2.0) * ad;
-            vd = vd + (dt / 2.0) * ad;                   // Integrate.cu:447-463
-            v.p[d][i] = vd;
-            u.p[d][i] += vd * dt;
+        double fd[DIM], bd[DIM], vd[DIM];
+        for (int d = 0; d < DIM; ++d) {
+            fd[d] = F32 ? (double)f32.p[d][i] : f.p[d][i];
+            bd[d] = b.p[d][i];
+            vd[d] = v.p[d][i];
         }
-    } else if (i < N_int + n_top) {
-        u.p[1][i] += load * dt;
-    } else if (i < N_int + n_top + n_bot) {
-        u.p[1][i] -= load * dt;
+        for (int d = 0; d < DIM; ++d) {
+            const double ad = (fd[d] + bd[d]) / dens;  // Integrate.cu:465-481
+            double w = vd[d] + (dt / 2.0) * ad;
+            w = w + (dt / 2.0) * ad;                    // Integrate.cu:447-463
+            v.p[d][i] = w;
+            ud[d] += w * dt;
+            u.p[d][i] = ud[d];
+        }
+    } else {
+        ud[1] = i < N_int + n_top ? ud[1] + load * dt : ud[1] - load * dt;
+        u.p[1][i] = ud[1];
     }
-    if (rec && i < N_int + n_top + n_bot)
-        store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
+    if (rec) store_rec_u(rec, i, ud[0], ud[1], ud[2]);
 }
 
 // Particles the fused LPS force pass does not visit (i >= N_int): the strip
@@ -181,17 +199,22 @@ __global__ void k_unpack_uv(int dim, int N, const double *__restrict__ rec, cons
     }
 }
 
-__global__ void k_verlet2(int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f, CF3 f32, int use32,
-                          CD3 b)
+template <int DIM, bool F32>
+__global__ void k_verlet2(int N_int, double dt, double dens, D3 v, D3 a, CD3 f, CF3 f32, CD3 b)
 {
     // Integrate.cu:465-481
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N_int) return;
-    for (int d = 0; d < dim; ++d) {
-        double fd = use32 ? (double)f32.p[d][i] : f.p[d][i];
-        double ad = (fd + b.p[d][i]) / dens;
+    double fd[DIM], bd[DIM], vd[DIM];
+    for (int d = 0; d < DIM; ++d) {
+        fd[d] = F32 ? (double)f32.p[d][i] : f.p[d][i];
+        bd[d] = b.p[d][i];
+        vd[d] = v.p[d][i];
+    }
+    for (int d = 0; d < DIM; ++d) {
+        const double ad = (fd[d] + bd[d]) / dens;
         a.p[d][i] = ad;
-        v.p[d][i] += (dt / 2.0) * ad;
+        v.p[d][i] = vd[d] + (dt / 2.0) * ad;
     }
 }
 
@@ -463,7 +486,10 @@ hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bo
 {
     int n = N_int + n_top + n_bot;
     if (n < 1) n = 1;
-    k_verlet1<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, load, v, a, u, cyl_state, rec);
+    if (dim == 3)
+        k_verlet1<3><<<nblk(n, 256), 256, 0, s>>>(N_int, n_top, n_bot, dt, load, v, a, u, cyl_state, rec);
+    else
+        k_verlet1<2><<<nblk(n, 256), 256, 0, s>>>(N_int, n_top, n_bot, dt, load, v, a, u, cyl_state, rec);
     return hipGetLastError();
 }
 
@@ -492,7 +518,18 @@ hipError_t launch_verlet21(hipStream_t s, int dim, int N_int, int n_top, int n_b
 {
     int n = N_int + n_top + n_bot;
     if (n < 1) n = 1;
-    k_verlet21<<<nblk(n, 256), 256, 0, s>>>(dim, N_int, n_top, n_bot, dt, dens, load, v, f, f32, use32, b, u, rec);
+    const dim3 g(nblk(n, 256)), t(256);
+    if (dim == 3) {
+        if (use32)
+            k_verlet21<3, true><<<g, t, 0, s>>>(N_int, n_top, n_bot, dt, dens, load, v, f, f32, b, u, rec);
+        else
+            k_verlet21<3, false><<<g, t, 0, s>>>(N_int, n_top, n_bot, dt, dens, load, v, f, f32, b, u, rec);
+    } else {
+        if (use32)
+            k_verlet21<2, true><<<g, t, 0, s>>>(N_int, n_top, n_bot, dt, dens, load, v, f, f32, b, u, rec);
+        else
+            k_verlet21<2, false><<<g, t, 0, s>>>(N_int, n_top, n_bot, dt, dens, load, v, f, f32, b, u, rec);
+    }
     return hipGetLastError();
 }
 
@@ -500,7 +537,18 @@ hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double d
                           CF3 f32, int use32, CD3 b)
 {
     if (N_int <= 0) return hipSuccess;
-    k_verlet2<<<nblk(N_int, 256), 256, 0, s>>>(dim, N_int, dt, dens, v, a, f, f32, use32, b);
+    const dim3 g(nblk(N_int, 256)), t(256);
+    if (dim == 3) {
+        if (use32)
+            k_verlet2<3, true><<<g, t, 0, s>>>(N_int, dt, dens, v, a, f, f32, b);
+        else
+            k_verlet2<3, false><<<g, t, 0, s>>>(N_int, dt, dens, v, a, f, f32, b);
+    } else {
+        if (use32)
+            k_verlet2<2, true><<<g, t, 0, s>>>(N_int, dt, dens, v, a, f, f32, b);
+        else
+            k_verlet2<2, false><<<g, t, 0, s>>>(N_int, dt, dens, v, a, f, f32, b);
+    }
     return hipGetLastError();
 }
 

@@ -56,41 +56,44 @@ __device__ __forceinline__ float group_sum(double a)
     return (float)a;
 }
 
-// Geometry of one bond in FP32 from FP64 differences.  Only this FP32 part
-// may contract a*b+c into FMA; the FP64 parity stretch below keeps the
-// separate roundings of the oracle.
+// Geometry of one bond in FP32 from FP64 differences.  Every fused
+// multiply-add is written out (the library is built without contraction), so
+// every inlined copy of a bond body rounds the same way, whichever kernel,
+// lane count or unrolling it sits in.  The FP64 parity stretch below keeps
+// the separate roundings of the oracle.
 template <int DIM>
 struct BondGeo {
     float dx, dy, dz;   // xi + eta
     float num;          // |xi+eta|^2 - |xi|^2 = eta.(2 xi + eta)
     float rL;           // 1/|xi+eta|
-    float L;            // |xi+eta|
+    float L2;           // |xi+eta|^2
     float xi2;          // |xi|^2
+    // |xi+eta| + |xi| for a reference length L0
+    __device__ __forceinline__ float len_sum(float L0) const { return __builtin_fmaf(L2, rL, L0); }
 };
 
 template <int DIM>
 __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
 {
-#pragma clang fp contract(fast)
     const float ex = (float)(rc.x0 - ri.x0), ey = (float)(rc.x1 - ri.x1);
     const float hx = (float)(rc.u0 - ri.u0), hy = (float)(rc.u1 - ri.u1);
     BondGeo<DIM> g;
     g.dx = ex + hx;
     g.dy = ey + hy;
-    float L2 = g.dx * g.dx + g.dy * g.dy;
-    g.num = hx * (2.f * ex + hx) + hy * (2.f * ey + hy);
-    g.xi2 = ex * ex + ey * ey;
+    float L2 = __builtin_fmaf(g.dy, g.dy, g.dx * g.dx);
+    g.num = __builtin_fmaf(hy, __builtin_fmaf(2.f, ey, hy), hx * __builtin_fmaf(2.f, ex, hx));
+    g.xi2 = __builtin_fmaf(ey, ey, ex * ex);
     if (DIM == 3) {
         const float ez = (float)(rc.x2 - ri.x2), hz = (float)(rc.u2 - ri.u2);
         g.dz = ez + hz;
-        L2 += g.dz * g.dz;
-        g.num += hz * (2.f * ez + hz);
-        g.xi2 += ez * ez;
+        L2 = __builtin_fmaf(g.dz, g.dz, L2);
+        g.num = __builtin_fmaf(hz, __builtin_fmaf(2.f, ez, hz), g.num);
+        g.xi2 = __builtin_fmaf(ez, ez, g.xi2);
     } else {
         g.dz = 0.f;
     }
+    g.L2 = L2;
     g.rL = frsq(L2);
-    g.L = L2 * g.rL;
     return g;
 }
 
@@ -103,7 +106,6 @@ __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
 template <bool W>
 __device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &L0, float &coef, float &rL0)
 {
-#pragma clang fp contract(fast)
     rL0 = frsq(xi2);
     L0 = xi2 * rL0;
     // 1 up to lo = delta - dx/2, (hi - |xi|)/dx up to hi = lo + dx, 0 beyond
