@@ -1,11 +1,14 @@
 """Physical observables of a run, computed from the engine's fields.
 
 crack_tip: the crack-tip tracker of the crack-growth driver
-(Solve_Plate_Crack_Grow.cu:150-162, commented out in the reference): on the
+(Solve_Plate_Crack_Grow.cu:150-160, commented out in the reference): on the
 centre row of an nx x ny plate, the first particle from the left edge whose
 damage exceeds 0.38 marks the tip of the left crack branch; its distance
-from the plate centre is the current crack half-length.  Sampling it every
-few steps gives the crack speed.
+from the plate centre is the current crack half-length.  (The reference
+writes (nx/2 - k - 50)*dx, the half-length less the 50 cells of its initial
+half-crack at dx = 0.1 mm, i.e. the extension; subtract the initial
+half-length from crack_half_length for that.)  Sampling it every few steps
+gives the crack speed.
 
 kw_crack_angles: the crack direction of each notch of the Kalthoff-Winkler
 specimen (notches cut along x = -rad and x = +rad for y >= 0,
