@@ -180,6 +180,8 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
             incs = []
             for k in range(batch_increments):
                 its, res, ms = b.increment(W.DU, 1e-6, max_iters, block=block)
+                if k == 0:
+                    parts_used = b.parts_used()
                 active = its > 0
                 unconv = int(((res > 1e-6) & active).sum())
                 nbs = [b.spec_info(s_)[2] for s_ in range(len(specs))] if k == 0 else None
@@ -204,7 +206,9 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                       "particles": n_part, "bonds": n_bonds, "plain_s": tot["plain"] / 1e3,
                       "warm_s": tot["warm"] / 1e3, "speedup": tot["plain"] / tot["warm"],
                       "increments_plain": runs["plain"], "increments_warm": runs["warm"],
-                      "launch_shape": {"block": block, "lanes_per_row": group}, "autotune_ms_per_iteration": tuned,
+                      "launch_shape": {"block": block, "lanes_per_row": group,
+                                       "workgroups_per_specimen": parts_used},
+                      "autotune_ms_per_iteration": tuned,
                       "increment0_bond_iterations_per_s": gpu_rate, "cpu_baseline": cpu}
     wpath = os.path.join(ROOT, "pd_gpu1_amd", "data", "mpnn_warmstart.npy")
     specs = W.eval_specs(n_eval)

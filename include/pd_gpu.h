@@ -225,6 +225,18 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
  * 4, 8, 16, 32 lanes per particle; every choice adds a row's bond terms in
  * bond order, so results are identical. */
 int pd_batch_set_group(pd_batch *b, int group);
+/* Workgroups per specimen (no reference counterpart): 1, 2, 4, 8 or 16 (at
+ * most block/64), or 0 (default) = the smallest power of two W at which the
+ * largest active specimen's work (bonds x sqrt(particles), as its ADR
+ * iterations grow with its side) over W workgroups is no more than all
+ * active work over the workgroups the device holds (the largest specimen
+ * then no longer sets the wall time alone).  The workgroups of a specimen
+ * meet at barriers in global memory and are all resident (the grid is
+ * capped); the damping sums keep the order of one workgroup, so results are
+ * identical for every choice.  pd_batch_get_parts reports the value the last
+ * increment ran with. */
+int pd_batch_set_parts(pd_batch *b, int parts);
+int pd_batch_get_parts(pd_batch *b, int *parts_used);
 /* Warm start of the batched increments (no reference counterpart): 0 = plain
  * ADR from the previous solution with the strips moved (default), 1 = the
  * free particles also move by the previous increment, which is exact while

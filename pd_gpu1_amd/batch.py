@@ -98,6 +98,17 @@ class Batch:
         """Lanes per particle of the force phase (1 = thread per particle)."""
         self._c(lib().pd_batch_set_group(self._h, group))
 
+    def set_parts(self, parts):
+        """Workgroups per specimen: 1, 2, 4, 8, 16, or 0 to size it from the
+        largest specimen's share of the work (same results)."""
+        self._c(lib().pd_batch_set_parts(self._h, parts))
+
+    def parts_used(self):
+        """Workgroups per specimen of the last increment."""
+        w = C.c_int()
+        self._c(lib().pd_batch_get_parts(self._h, C.byref(w)))
+        return w.value
+
     def set_warm(self, mode):
         """1: start each increment from u + the previous increment (0: plain)."""
         self._c(lib().pd_batch_set_warm(self._h, mode))
@@ -154,6 +165,7 @@ def main(argv=None):
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--grid", type=int, default=0, help="workgroups (0: as many as are resident at once)")
     ap.add_argument("--group", type=int, default=4)
+    ap.add_argument("--parts", type=int, default=0, help="workgroups per specimen (0: sized from the work)")
     ap.add_argument("--warm", type=int, default=0)
     ap.add_argument("--retire", type=int, default=1)
     a = ap.parse_args(argv)
@@ -162,6 +174,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     with Batch(plate_params(), specs) as b:
         b.set_group(a.group)
+        b.set_parts(a.parts)
         b.set_warm(a.warm)
         b.set_retire(a.retire)
         n_part, n_bonds, _ = b.info()
@@ -169,7 +182,7 @@ def main(argv=None):
         for k in range(a.increments):
             its, res, ms = b.increment(a.du, a.tol, a.max_iters, block=a.block, grid=a.grid)
             act = its > 0
-            rows.append({"increment": k, "ms": round(ms, 1), "active": int(act.sum()),
+            rows.append({"increment": k, "ms": round(ms, 1), "active": int(act.sum()), "parts": b.parts_used(),
                          "iterations_median": int(np.median(its[act])) if act.any() else 0,
                          "iterations_max": int(its.max()), "unconverged": int(((res > a.tol) & act).sum()),
                          "specimen_iterations": int(its.sum())})

@@ -15,6 +15,15 @@
 // phases; no grid-wide synchronisation and no host round trip is needed
 // (the reference copies both sums to the host every iteration,
 // Reduce.cu:61-62, Integrate.cu:112-113).
+//
+// When there are fewer specimens than resident workgroups, W > 1 workgroups
+// (a group, W a power of two) share each specimen: rows and particles are
+// split over the group's W*BT threads, the phases are separated by a group
+// barrier in global memory (fence, arrival counter, bounded spin, fence), and
+// the damping sums keep the single-workgroup order: part p of the group owns
+// the T/W consecutive lanes p*T/W ... of the T = BT lane tree, a complete
+// subtree, and the W subtree roots are combined by the remaining levels of
+// the same tree.  Results are therefore identical for every W.
 #include "pd_batch.h"
 
 namespace {
@@ -74,26 +83,78 @@ __device__ __forceinline__ double uniform_double(double v)
     return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_readfirstlane(w.x), __builtin_amdgcn_readfirstlane(w.y)));
 }
 
+// Barrier of the W workgroups of a group.  Every workgroup of the grid is
+// resident (the host sizes the grid from the occupancy), so the spin ends
+// when the group's other workgroups arrive; a wait longer than kSpinTicks of
+// the constant-rate wall clock, or another group's abort, sets the abort
+// flag and the workgroup leaves the kernel.  Returns false on abort.
+constexpr uint64_t kSpinTicks = 2000000000ull;  // 20 s at the 100 MHz wall clock
+
+__device__ __forceinline__ bool group_sync(const BatchGroups &gr, int grp, unsigned &arrivals, int &ok_sh)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        arrivals += (unsigned)gr.parts;
+        __threadfence();  // this workgroup's writes before its arrival
+        atomicAdd(&gr.bar[grp], 1u);
+        const uint64_t t0 = wall_clock64();
+        int ok = 1;
+        while (__hip_atomic_load(&gr.bar[grp], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < arrivals) {
+            if (__hip_atomic_load(gr.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                wall_clock64() - t0 > kSpinTicks) {
+                atomicExch(gr.abort, 1);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();  // the group's writes before this workgroup's reads
+        ok_sh = ok;
+    }
+    __syncthreads();
+    return ok_sh != 0;
+}
+
 template <int BT, int G>
 __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double tol, int max_iters,
-                                                  unsigned *queue, int32_t *iters_out, double *resid_out)
+                                                  unsigned *queue, int32_t *iters_out, double *resid_out,
+                                                  BatchGroups gr)
 {
     __shared__ double sh[4][BT / 64];
-    __shared__ int spec_sh;
+    __shared__ int spec_sh, ok_sh;
     __shared__ double c_sh, r_sh;
+    __shared__ double q_sh[kMaxParts][4];
     const int tid = threadIdx.x;
+    const int W = gr.parts;                       // workgroups per specimen
+    const int grp = (int)blockIdx.x / W, part = (int)blockIdx.x % W;
+    const int gt = part * BT + tid, GT = W * BT;  // thread within the group, threads of the group
+    const int L = BT / W;                         // damping-tree lanes of this part
+    unsigned arrivals = 0;
+#define PD_GROUP_SYNC()                                                        \
+    do {                                                                       \
+        if (W == 1)                                                            \
+            __syncthreads();                                                   \
+        else if (!group_sync(gr, grp, arrivals, ok_sh))                        \
+            return;                                                            \
+    } while (0)
     for (;;) {
-        if (tid == 0) {
+        if (tid == 0 && part == 0) {
             const int t = (int)atomicAdd(queue, 1u);
-            spec_sh = t < B.nspec ? B.order[t] : B.nspec;
+            const int q = t < B.nspec ? B.order[t] : B.nspec;
+            if (W == 1)
+                spec_sh = q;
+            else
+                gr.spec[grp] = q;
         }
+        PD_GROUP_SYNC();
+        if (W > 1 && tid == 0) spec_sh = gr.spec[grp];
         __syncthreads();
         const int s = uniform_int(spec_sh);
-        __syncthreads();
+        PD_GROUP_SYNC();  // every part has read s before part 0 takes the next ticket
         if (s >= B.nspec) break;
         if (B.retired && B.retired[s]) {
-            if (tid == 0) iters_out[s] = 0;
-            continue;  // uniform: every thread read the same s
+            if (tid == 0 && part == 0) iters_out[s] = 0;
+            continue;  // uniform: every thread of the group read the same s
         }
         const BatchSpec d = B.spec[s];
         const int N = d.n, Ni = d.n_int;
@@ -113,7 +174,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         double2 *P = B.pos + d.p_off;
         if (B.warm) {
             // start from u + the previous increment; keep the start state
-            for (int i = tid; i < Ni; i += BT) {
+            for (int i = gt; i < Ni; i += GT) {
                 const double sx = UX[i], sy = UY[i];
                 UX[i] = sx + DUPX[i];
                 UY[i] = sy + DUPY[i];
@@ -122,16 +183,16 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
             }
         }
         // displacement control of the strips (Solve_Plate_Crack_Grow.cu:20-27 layout)
-        for (int k = tid; k < d.n_top + d.n_bot; k += BT) UY[Ni + k] += k < d.n_top ? du : -du;
-        __syncthreads();
-        for (int i = tid; i < N; i += BT) P[i] = make_double2(X[i] + UX[i], Y[i] + UY[i]);
-        __syncthreads();
+        for (int k = gt; k < d.n_top + d.n_bot; k += GT) UY[Ni + k] += k < d.n_top ? du : -du;
+        PD_GROUP_SYNC();
+        for (int i = gt; i < N; i += GT) P[i] = make_double2(X[i] + UX[i], Y[i] + UY[i]);
+        PD_GROUP_SYNC();
         int it = 0;
         double r = 0.0;
         for (;;) {
             // PMB force, every row (Force_Bond.cu:169)
             if (G == 1) {
-                for (int i = tid; i < N; i += BT) {
+                for (int i = gt; i < N; i += GT) {
                     const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
                     double fx = 0.0, fy = 0.0;
                     for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
@@ -146,7 +207,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                 }
             } else {
                 const int lane = tid & (G - 1);
-                for (int i = tid / G; i < N; i += BT / G) {  // the whole group takes the row
+                for (int i = gt / G; i < N; i += GT / G) {  // the lane group takes the row
                     const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
                     double fx = 0.0, fy = 0.0;
                     const int32_t kb = rp[i], ke = rp[i + 1];
@@ -182,10 +243,11 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                     }
                 }
             }
-            __syncthreads();
-            // damping and residual terms, lane t owns particles t, t+BT, ...
+            PD_GROUP_SYNC();
+            // damping and residual terms, lane t of the T = BT lane tree owns
+            // particles t, t+BT, ...; this part holds lanes part*L ... part*L+L-1
             double p[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int i = tid; i < N; i += BT) {
+            for (int i = tid < L ? part * L + tid : N; i < N; i += BT) {
                 double cn = 0.0, disp = 0.0, rn = 0.0, rd = 0.0;
                 if (i < Ni) {
                     const double ux = UX[i], uy = UY[i], fx = FX[i], fy = FY[i];
@@ -205,7 +267,23 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                 p[3] += rd;
             }
             double tot[4];
+            // lanes from L on add zeros, so with W > 1 the workgroup's tree
+            // returns the root of this part's subtree exactly
             block_canon4<BT>(p, sh, tot);
+            if (W > 1) {
+                if (tid == 0)
+                    for (int c = 0; c < 4; ++c) gr.sum[((size_t)grp * W + part) * 4 + c] = tot[c];
+                PD_GROUP_SYNC();
+                if (tid == 0) {
+                    // the remaining levels of the tree, over the W subtree roots
+                    for (int w = 0; w < W; ++w)
+                        for (int c = 0; c < 4; ++c) q_sh[w][c] = gr.sum[((size_t)grp * W + w) * 4 + c];
+                    for (int off = 1; off < W; off <<= 1)
+                        for (int w = 0; w < W; w += 2 * off)
+                            for (int c = 0; c < 4; ++c) q_sh[w][c] = q_sh[w][c] + q_sh[w + off][c];
+                    for (int c = 0; c < 4; ++c) tot[c] = q_sh[0][c];
+                }
+            }
             if (tid == 0) {
                 double cn1 = tot[0], cn2 = tot[1], cc;
                 if (cn2 != 0.0) {
@@ -224,7 +302,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
             r = uniform_double(r_sh);
             if (r <= tol || it >= max_iters || !(r == r)) break;
             const double cc = uniform_double(c_sh);
-            for (int i = tid; i < Ni; i += BT) {
+            for (int i = gt; i < Ni; i += GT) {
                 double h0, h1;
                 const double fx = FX[i], fy = FY[i];
                 if (it == 0) {
@@ -247,22 +325,23 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                 FOX[i] = fx;
                 FOY[i] = fy;
             }
-            __syncthreads();
+            PD_GROUP_SYNC();
             ++it;
         }
         if (B.warm) {
-            for (int i = tid; i < Ni; i += BT) {
+            for (int i = gt; i < Ni; i += GT) {
                 DUPX[i] = UX[i] - DUPX[i];
                 DUPY[i] = UY[i] - DUPY[i];
             }
         }
-        if (tid == 0) {
+        if (tid == 0 && part == 0) {
             iters_out[s] = it;
             resid_out[s] = r;
             if (B.retired && !(r <= tol)) B.retired[s] = 1;
         }
-        __syncthreads();
+        PD_GROUP_SYNC();
     }
+#undef PD_GROUP_SYNC
 }
 
 } // namespace
@@ -277,14 +356,14 @@ __global__ void k_pack_bond_pairs(int64_t nb, const double *__restrict__ kappa, 
 
 template <int BT>
 hipError_t batch_g(hipStream_t st, int grid, int group, const BatchView &B, double du, double tol, int max_iters,
-                   unsigned *queue, int32_t *iters_out, double *resid_out)
+                   unsigned *queue, int32_t *iters_out, double *resid_out, const BatchGroups &gr)
 {
     switch (group) {
-    case 1: k_batch_adr<BT, 1><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
-    case 4: k_batch_adr<BT, 4><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
-    case 8: k_batch_adr<BT, 8><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
-    case 16: k_batch_adr<BT, 16><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
-    case 32: k_batch_adr<BT, 32><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out); break;
+    case 1: k_batch_adr<BT, 1><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
+    case 4: k_batch_adr<BT, 4><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
+    case 8: k_batch_adr<BT, 8><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
+    case 16: k_batch_adr<BT, 16><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
+    case 32: k_batch_adr<BT, 32><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -315,12 +394,17 @@ int batch_resident_per_cu(int block, int group)
 }
 
 hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
-                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out)
+                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out,
+                            const BatchGroups &gr)
 {
+    // W must divide the grid and the damping tree (block/64 waves)
+    if (gr.parts < 1 || gr.parts > kMaxParts || (gr.parts & (gr.parts - 1)) || block / 64 < gr.parts ||
+        grid % gr.parts || (gr.parts > 1 && (!gr.bar || !gr.spec || !gr.sum || !gr.abort)))
+        return hipErrorInvalidValue;
     switch (block) {
-    case 256: return batch_g<256>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out);
-    case 512: return batch_g<512>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out);
-    case 1024: return batch_g<1024>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out);
+    case 256: return batch_g<256>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    case 512: return batch_g<512>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    case 1024: return batch_g<1024>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
     default: return hipErrorInvalidValue;
     }
 }

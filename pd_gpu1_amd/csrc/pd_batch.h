@@ -26,8 +26,20 @@ struct BatchView {
     double mass, dt, s0;
 };
 
+// Several workgroups per specimen (k_batch_adr): workgroups per group, and
+// per group an arrival counter, the specimen slot and the parts' damping sums.
+constexpr int kMaxParts = 16;
+struct BatchGroups {
+    int parts;      // workgroups per specimen: a power of two <= min(16, block/64)
+    unsigned *bar;  // [groups] arrival counters, zero at launch
+    int *spec;      // [groups] the group's current specimen
+    double *sum;    // [groups][parts][4] subtree roots of the damping tree
+    int *abort;     // set when a group barrier times out
+};
+
 hipError_t launch_pack_bond_pairs(hipStream_t st, int64_t nb, const double *kappa, const double *idist,
                                   double2 *kd);
 int batch_resident_per_cu(int block, int group);
 hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
-                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out);
+                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out,
+                            const BatchGroups &gr);

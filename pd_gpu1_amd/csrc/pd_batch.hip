@@ -9,6 +9,7 @@
 #include "pd_batch.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -33,6 +34,13 @@ struct pd_batch {
     int retire = 0;
     int last_block = 1024;
     int group = 1;                // lanes per row of the force phase (1 = thread per particle)
+    int parts = 0;                // workgroups per specimen (0 = sized from the batch's work)
+    int last_parts = 1;           // the value the last increment ran with
+    int groups_cap = 0;           // groups the group buffers hold
+    unsigned *g_bar = nullptr;    // [groups_cap] barrier arrival counters
+    int *g_spec = nullptr;        // [groups_cap] specimen of each group
+    double *g_sum = nullptr;      // [groups_cap][kMaxParts][4] damping subtree roots
+    int *g_abort = nullptr;       // a group barrier timed out
     int warm = 0;                 // 1: increments start from u + the previous increment
     BatchView view() const
     {
@@ -67,7 +75,7 @@ struct pd_batch {
 static void batch_free(pd_batch *b)
 {
     void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->fail, b->queue, b->d_iters,
-                    b->d_resid, b->d_retired};
+                    b->d_resid, b->d_retired, b->g_bar, b->g_spec, b->g_sum, b->g_abort};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -238,27 +246,83 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     }
     B_TRY(hipSetDevice(b->device));
     const int nspec = (int)b->spec.size();
-    if (grid <= 0) {
-        // every workgroup resident at once (a persistent grid)
-        int dev_cu = 0;
-        B_TRY(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, b->device));
-        grid = dev_cu * batch_resident_per_cu(block, b->group);
+    // every workgroup resident at once (a persistent grid): the capacity
+    int dev_cu = 0;
+    B_TRY(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, b->device));
+    const int capacity = dev_cu * batch_resident_per_cu(block, b->group);
+    int W = b->parts;
+    const int wmax = std::min(kMaxParts, block / 64);
+    if (W == 0) {
+        // A specimen's work is its bonds times its ADR iterations, which grow
+        // with the plate's side (sqrt of its particles).  The largest
+        // specimen bounds the wall time from below by its work over W
+        // workgroups, the whole batch by all work over the device's
+        // workgroups: take the smallest power of two W that brings the first
+        // bound down to the second.
+        std::vector<int32_t> ret((size_t)nspec, 0);
+        if (b->retire)
+            B_TRY(hipMemcpy(ret.data(), b->d_retired, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
+        double total = 0.0, largest = 0.0;
+        for (int q = 0; q < nspec; ++q)
+            if (!ret[q]) {
+                const double work = (double)b->spec[q].nb * std::sqrt((double)b->spec[q].n);
+                total += work;
+                largest = std::max(largest, work);
+            }
+        W = 1;
+        while (2 * W <= wmax && largest * capacity > total * W) W *= 2;
+    } else if (W > wmax) {
+        b->err = "workgroups per specimen exceed block/64 (the damping tree's waves)";
+        return PD_E_INVALID;
     }
-    if (grid > nspec) grid = nspec;
+    int groups = grid > 0 ? grid / W : capacity / W;
+    if (W > 1 && groups > capacity / W) groups = capacity / W;  // a group's workgroups must all be resident
+    if (groups > nspec) groups = nspec;
+    if (groups < 1) {
+        b->err = "grid holds fewer workgroups than one specimen's group";
+        return PD_E_INVALID;
+    }
+    BatchGroups gr{W, nullptr, nullptr, nullptr, nullptr};
+    if (W > 1) {
+        if (groups > b->groups_cap) {
+            void *old[] = {b->g_bar, b->g_spec, b->g_sum, b->g_abort};
+            for (void *q : old)
+                if (q) (void)hipFree(q);
+            b->g_bar = nullptr; b->g_spec = nullptr; b->g_sum = nullptr; b->g_abort = nullptr;
+            b->groups_cap = 0;
+            B_TRY(hipMalloc((void **)&b->g_bar, sizeof(unsigned) * groups));
+            B_TRY(hipMalloc((void **)&b->g_spec, sizeof(int) * groups));
+            B_TRY(hipMalloc((void **)&b->g_sum, sizeof(double) * 4 * kMaxParts * groups));
+            B_TRY(hipMalloc((void **)&b->g_abort, sizeof(int)));
+            b->groups_cap = groups;
+        }
+        gr = BatchGroups{W, b->g_bar, b->g_spec, b->g_sum, b->g_abort};
+        B_TRY(hipMemsetAsync(b->g_bar, 0, sizeof(unsigned) * groups, b->stream));
+        B_TRY(hipMemsetAsync(b->g_abort, 0, sizeof(int), b->stream));
+    }
     hipEvent_t e0, e1;
     B_TRY(hipEventCreate(&e0));
     B_TRY(hipEventCreate(&e1));
     B_TRY(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
     B_TRY(hipEventRecord(e0, b->stream));
-    B_TRY(launch_batch_adr(b->stream, grid, block, b->group, b->view(), du, tol, max_iters, b->queue, b->d_iters,
-                           b->d_resid));
+    B_TRY(launch_batch_adr(b->stream, groups * W, block, b->group, b->view(), du, tol, max_iters, b->queue,
+                           b->d_iters, b->d_resid, gr));
     B_TRY(hipEventRecord(e1, b->stream));
     B_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
     B_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (W > 1) {
+        int aborted = 0;
+        B_TRY(hipMemcpy(&aborted, b->g_abort, sizeof(int), hipMemcpyDeviceToHost));
+        if (aborted) {
+            b->err = "a workgroup group barrier timed out (the group's workgroups were not all resident)";
+            return PD_E_HIP;
+        }
+    }
     b->last_block = block;
+    b->last_parts = W;
     if (ms_out) *ms_out = ms;
     if (iters_out) B_TRY(hipMemcpy(iters_out, b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
     if (resid_out) B_TRY(hipMemcpy(resid_out, b->d_resid, sizeof(double) * nspec, hipMemcpyDeviceToHost));
@@ -282,6 +346,28 @@ int pd_batch_set_retire(pd_batch *b, int on)
     // skip it and report 0 iterations for it.
     if (!b || (on != 0 && on != 1)) return PD_E_INVALID;
     b->retire = on;
+    return PD_OK;
+}
+
+int pd_batch_set_parts(pd_batch *b, int parts)
+{
+    // Workgroups per specimen: 1, 2, 4, 8 or 16 (at most block/64), or 0
+    // (default) for the smallest power of two at which the largest active
+    // specimen's share of the device (its work over W workgroups) is no
+    // longer than the whole batch's.  Results are identical for every choice.
+    if (!b) return PD_E_INVALID;
+    if (parts < 0 || parts > kMaxParts || (parts & (parts - 1))) {
+        b->err = "parts must be 0, 1, 2, 4, 8 or 16";
+        return PD_E_INVALID;
+    }
+    b->parts = parts;
+    return PD_OK;
+}
+
+int pd_batch_get_parts(pd_batch *b, int *parts_used)
+{
+    if (!b || !parts_used) return PD_E_INVALID;
+    *parts_used = b->last_parts;
     return PD_OK;
 }
 

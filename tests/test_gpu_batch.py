@@ -18,8 +18,9 @@ def oracle_increment(m, du, tol, max_iters, T):
     return adr_increment(m, du, tol, max_iters, T)
 
 
-@pytest.mark.parametrize("block,group", [(256, 1), (1024, 1), (256, 8), (1024, 32)])
-def test_batch_adr_bit_exact_against_oracle(block, group):
+@pytest.mark.parametrize("block,group,parts", [(256, 1, 1), (1024, 1, 1), (256, 8, 1), (1024, 32, 1),
+                                              (256, 1, 4), (1024, 4, 2), (1024, 8, 16)])
+def test_batch_adr_bit_exact_against_oracle(block, group, parts):
     p = PB.plate_params()
     specs = [(12, 12, 0.0025, 0.3), (17, 17, 0.003, -0.2), (9, 9, 0.0, 0.0), (21, 21, 0.004, 0.5),
              (14, 14, 0.002, 0.0)]
@@ -27,11 +28,13 @@ def test_batch_adr_bit_exact_against_oracle(block, group):
     du, tol, max_it = 3.0e-6, 1e-6, 600
     with PB.Batch(p, specs) as b:
         b.set_group(group)
+        b.set_parts(parts)
         for s, m in enumerate(ms):
             rp, col, mask = b.csr(s)
             assert np.array_equal(rp, m.row_ptr) and np.array_equal(col, m.col) and np.array_equal(mask, m.fail)
         for inc in range(3):
-            its, res, _ = b.increment(du, tol, max_it, block=block, grid=3)
+            its, res, _ = b.increment(du, tol, max_it, block=block, grid=3 * parts)
+            assert b.parts_used() == parts
             for s, m in enumerate(ms):
                 it_o, r_o = oracle_increment(m, du, tol, max_it, block)
                 assert its[s] == it_o and res[s] == r_o, (inc, s, its[s], it_o, res[s], r_o)
@@ -70,11 +73,32 @@ def test_batch_warm_start_reaches_the_same_states_in_fewer_iterations():
     assert sum(x.sum() for x in its_w[1:]) < 0.5 * sum(x.sum() for x in its_p[1:])
 
 
+def test_batch_parts_match_one_workgroup_per_specimen():
+    # several workgroups per specimen (chosen to fill the device) against one,
+    # with the warm start: the same iterations, residuals, states and bonds
+    p = PB.plate_params()
+    specs = PB.plate_family(12, n_min=24, n_max=60)
+    with PB.Batch(p, specs) as one, PB.Batch(p, specs) as many:
+        one.set_parts(1)
+        many.set_parts(0)
+        for b in (one, many):
+            b.set_warm(1)
+        for inc in range(3):
+            a, ra, _ = one.increment(1.0e-6, 1e-6, 200000, block=1024)
+            b, rb, _ = many.increment(1.0e-6, 1e-6, 200000, block=1024)
+            assert many.parts_used() > 1 and one.parts_used() == 1
+            assert np.array_equal(a, b) and np.array_equal(ra, rb), inc
+        for s in range(len(specs)):
+            assert np.array_equal(one.get(s, "U"), many.get(s, "U")), s
+            assert np.array_equal(one.csr(s)[2], many.csr(s)[2]), s
+
+
 def test_batch_retire_skips_only_unconverged_specimens():
     p = PB.plate_params()
     specs = PB.plate_family(6, n_min=24, n_max=40)
     with PB.Batch(p, specs) as cut, PB.Batch(p, specs) as ok:
         cut.set_retire(1)
+        cut.set_parts(0)  # the skip runs in every workgroup of a specimen's group
         ok.set_retire(1)
         a, ra, _ = cut.increment(1.0e-6, 1e-6, 20, block=1024)  # 20 iterations reach no specimen's tol
         assert np.all(a == 20) and np.all(ra > 1e-6)
