@@ -1,9 +1,10 @@
 // k_batch.hip -- batched quasi-static crack growth: many independent 2D PMB
 // plates relaxed by adaptive dynamic relaxation (ADR) in one persistent launch.
 //
-// One workgroup owns one specimen at a time and pulls the next specimen index
-// from a device work queue (an atomic counter over the specimens sorted by
-// bond count, largest first), so 512 plates of 1.6k-40k particles
+// One workgroup (or a group of W, below) owns one specimen at a time and
+// pulls the next specimen index from a device work queue (an atomic counter
+// over the specimens sorted by bond count, largest first), so 512 plates of
+// 1.6k-40k particles
 // load-balance over the 256 CUs without host involvement and the largest
 // plates, which set the wall time, start first.  Inside a
 // specimen the workgroup runs whole ADR iterations: PMB force over every row
