@@ -237,6 +237,11 @@ int pd_batch_set_group(pd_batch *b, int group);
  * increment ran with. */
 int pd_batch_set_parts(pd_batch *b, int parts);
 int pd_batch_get_parts(pd_batch *b, int *parts_used);
+/* Longest wait (seconds, default 20) of a workgroup at a group barrier; a
+ * longer wait abandons the increment: every workgroup leaves the kernel and
+ * pd_batch_increment returns PD_E_HIP with the reason in
+ * pd_batch_last_error (no reference counterpart). */
+int pd_batch_set_barrier_timeout(pd_batch *b, double seconds);
 /* Warm start of the batched increments (no reference counterpart): 0 = plain
  * ADR from the previous solution with the strips moved (default), 1 = the
  * free particles also move by the previous increment, which is exact while

@@ -103,6 +103,10 @@ class Batch:
         largest specimen's share of the work (same results)."""
         self._c(lib().pd_batch_set_parts(self._h, parts))
 
+    def set_barrier_timeout(self, seconds):
+        """Longest wait at a group barrier before an increment is abandoned."""
+        self._c(lib().pd_batch_set_barrier_timeout(self._h, seconds))
+
     def parts_used(self):
         """Workgroups per specimen of the last increment."""
         w = C.c_int()

@@ -86,11 +86,10 @@ __device__ __forceinline__ double uniform_double(double v)
 
 // Barrier of the W workgroups of a group.  Every workgroup of the grid is
 // resident (the host sizes the grid from the occupancy), so the spin ends
-// when the group's other workgroups arrive; a wait longer than kSpinTicks of
-// the constant-rate wall clock, or another group's abort, sets the abort
-// flag and the workgroup leaves the kernel.  Returns false on abort.
-constexpr uint64_t kSpinTicks = 2000000000ull;  // 20 s at the 100 MHz wall clock
-
+// when the group's other workgroups arrive; a wait longer than
+// gr.spin_ticks of the constant-rate wall clock, or another group's abort,
+// sets the abort flag and the workgroup leaves the kernel.  Returns false on
+// abort.
 __device__ __forceinline__ bool group_sync(const BatchGroups &gr, int grp, unsigned &arrivals, int &ok_sh)
 {
     __syncthreads();
@@ -102,7 +101,7 @@ __device__ __forceinline__ bool group_sync(const BatchGroups &gr, int grp, unsig
         int ok = 1;
         while (__hip_atomic_load(&gr.bar[grp], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < arrivals) {
             if (__hip_atomic_load(gr.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                wall_clock64() - t0 > kSpinTicks) {
+                wall_clock64() - t0 > gr.spin_ticks) {
                 atomicExch(gr.abort, 1);
                 ok = 0;
                 break;

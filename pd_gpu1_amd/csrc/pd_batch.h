@@ -35,6 +35,7 @@ struct BatchGroups {
     int *spec;      // [groups] the group's current specimen
     double *sum;    // [groups][parts][4] subtree roots of the damping tree
     int *abort;     // set when a group barrier times out
+    uint64_t spin_ticks;  // longest wait at a group barrier, in wall-clock ticks
 };
 
 hipError_t launch_pack_bond_pairs(hipStream_t st, int64_t nb, const double *kappa, const double *idist,

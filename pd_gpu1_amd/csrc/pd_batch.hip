@@ -41,6 +41,7 @@ struct pd_batch {
     int *g_spec = nullptr;        // [groups_cap] specimen of each group
     double *g_sum = nullptr;      // [groups_cap][kMaxParts][4] damping subtree roots
     int *g_abort = nullptr;       // a group barrier timed out
+    double barrier_timeout = 20.0;  // seconds a workgroup waits at a group barrier
     int warm = 0;                 // 1: increments start from u + the previous increment
     BatchView view() const
     {
@@ -282,7 +283,11 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         b->err = "grid holds fewer workgroups than one specimen's group";
         return PD_E_INVALID;
     }
-    BatchGroups gr{W, nullptr, nullptr, nullptr, nullptr};
+    int clock_khz = 0;  // the wall clock that bounds a barrier wait
+    B_TRY(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, b->device));
+    const double ticks = b->barrier_timeout * (clock_khz > 0 ? clock_khz : 100000) * 1e3;
+    const uint64_t spin = ticks < 1.0 ? 1ull : ticks > 1e18 ? (uint64_t)1e18 : (uint64_t)ticks;
+    BatchGroups gr{W, nullptr, nullptr, nullptr, nullptr, spin};
     if (W > 1) {
         if (groups > b->groups_cap) {
             void *old[] = {b->g_bar, b->g_spec, b->g_sum, b->g_abort};
@@ -296,7 +301,7 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
             B_TRY(hipMalloc((void **)&b->g_abort, sizeof(int)));
             b->groups_cap = groups;
         }
-        gr = BatchGroups{W, b->g_bar, b->g_spec, b->g_sum, b->g_abort};
+        gr = BatchGroups{W, b->g_bar, b->g_spec, b->g_sum, b->g_abort, spin};
         B_TRY(hipMemsetAsync(b->g_bar, 0, sizeof(unsigned) * groups, b->stream));
         B_TRY(hipMemsetAsync(b->g_abort, 0, sizeof(int), b->stream));
     }
@@ -361,6 +366,15 @@ int pd_batch_set_parts(pd_batch *b, int parts)
         return PD_E_INVALID;
     }
     b->parts = parts;
+    return PD_OK;
+}
+
+int pd_batch_set_barrier_timeout(pd_batch *b, double seconds)
+{
+    // Longest wait of a workgroup at a group barrier before the increment is
+    // abandoned with an error (default 20 s).
+    if (!b || !(seconds > 0.0)) return PD_E_INVALID;
+    b->barrier_timeout = seconds;
     return PD_OK;
 }
 

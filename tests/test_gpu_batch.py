@@ -5,6 +5,7 @@ import pytest
 
 from pd_gpu1_amd import batch as PB
 from pd_gpu1_amd import scenarios
+from pd_gpu1_amd._lib import PdError
 from oracle.runner import adr_increment, plate_specimen
 
 pytestmark = pytest.mark.gpu
@@ -110,3 +111,19 @@ def test_batch_retire_skips_only_unconverged_specimens():
         for inc in range(2):  # converged increments never retire a specimen
             b, rb, _ = ok.increment(1.0e-6, 1e-6, 200000, block=1024)
             assert np.all(rb <= 1e-6) and np.all(b > 0)
+
+
+def test_batch_group_barrier_timeout_is_reported():
+    # a one-tick bound on the group barriers' wait: every workgroup leaves the
+    # kernel at its next wait and the increment reports the timeout
+    p = PB.plate_params()
+    specs = PB.plate_family(8, n_min=40, n_max=80)
+    with PB.Batch(p, specs) as b:
+        b.set_parts(4)
+        b.set_barrier_timeout(1e-9)
+        with pytest.raises(PdError, match="timed out"):
+            b.increment(1.0e-6, 1e-6, 2000, block=1024)
+    with PB.Batch(p, specs) as b:  # the default bound: the same batch converges
+        b.set_parts(4)
+        its, res, _ = b.increment(1.0e-6, 1e-6, 200000, block=1024)
+        assert b.parts_used() == 4 and np.all(res <= 1e-6)
