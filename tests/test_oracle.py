@@ -365,6 +365,53 @@ def test_sum_tree_order_and_value():
     assert math.isclose(O.sum_tree(v, 64), math.fsum(v), rel_tol=1e-10, abs_tol=1e-6)
 
 
+def _tree_split_py(v, T, W, sequential_roots=False):
+    """The T-lane tree as W workgroups of a batched specimen form it: part w
+    sums the chains of lanes w*T/W ... (w+1)*T/W - 1, reduces them with the
+    low-bit-first tree (its other lanes hold zeros), and the W subtree roots
+    are combined by the remaining tree levels."""
+    L = T // W
+    chains = [0.0] * T
+    for t in range(T):
+        sc = 0.0
+        for k in range(t, len(v), T):
+            sc += float(v[k])
+        chains[t] = sc
+    roots = []
+    for w in range(W):
+        p = chains[w * L:(w + 1) * L] + [0.0] * (T - L)
+        off = 1
+        while off < T:
+            for t in range(0, T, 2 * off):
+                p[t] = p[t] + p[t + off]
+            off *= 2
+        roots.append(p[0])
+    if sequential_roots:
+        r = roots[0]
+        for x in roots[1:]:
+            r = r + x
+        return r
+    off = 1
+    while off < W:
+        for w in range(0, W, 2 * off):
+            roots[w] = roots[w] + roots[w + off]
+        off *= 2
+    return roots[0]
+
+
+def test_sum_tree_split_over_workgroups_is_exact():
+    # the batched kernel's W workgroups per specimen reproduce the T-lane
+    # tree bit for bit; combining the roots in any other order does not
+    rng = np.random.default_rng(7)
+    differs = 0
+    for trial in range(20):
+        v = rng.standard_normal(4000 + trial) * 10.0 ** rng.integers(-6, 6, 4000 + trial)
+        for T, W in ((256, 2), (256, 4), (1024, 8), (1024, 16)):
+            assert _tree_split_py(v, T, W) == O.sum_tree(v, T), (trial, T, W)
+        differs += _tree_split_py(v, 1024, 16, sequential_roots=True) != O.sum_tree(v, 1024)
+    assert differs > 0
+
+
 def test_damage_closed_form():
     m, p = lattice_model(30, 30)
     m.init_state()
