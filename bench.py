@@ -385,7 +385,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py: the engine runs one specimen on one GPU; --gpus above 1 is not supported "
+        print("bench.py: the engine runs on one GPU; --gpus above 1 is not supported "
               "(sharding across GPUs exists as a design only).  Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
