@@ -949,8 +949,9 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
     REQUIRE(h->col, "pd_build_neighbors first");
     int rc = ensure_coef(h);
     if (rc) return rc;
-    hipEvent_t e[3];
-    for (auto &ev : e) LAUNCH(hipEventCreate(&ev));
+    HipEvents<3> events;
+    LAUNCH(events.create());
+    hipEvent_t *e = events.e;
     const pd_params &P = h->P;
     const double ymin = P.dim == 3 ? P.break_y_min : -INFINITY;
     double acc[2] = {0.0, 0.0};
@@ -1010,7 +1011,6 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
         acc[0] += t0;
         acc[1] += t1;
     }
-    for (auto &ev : e) (void)hipEventDestroy(ev);
     ms_out[0] = acc[0] / reps;
     ms_out[1] = acc[1] / reps;
     return rc;
@@ -1040,9 +1040,11 @@ int pd_stream_copy(int device, int64_t bytes, int reps, double *gbps)
         (void)hipFree(a);
         return PD_E_HIP;
     }
-    hipEvent_t e0, e1;
-    int rc = PD_OK;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PD_E_HIP;
+    HipEvents<2> ev;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ev.create() == hipSuccess ? PD_OK : PD_E_HIP;
+    e0 = ev.e[0];
+    e1 = ev.e[1];
     if (!rc && hipMemset(a, 1, bytes) != hipSuccess) rc = PD_E_HIP;
     if (!rc && launch_copy16(0, a, b, bytes) != hipSuccess) rc = PD_E_HIP;  // warm
     if (!rc) {
@@ -1055,8 +1057,6 @@ int pd_stream_copy(int device, int64_t bytes, int reps, double *gbps)
             rc = PD_E_HIP;
         if (!rc) *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
     return rc;

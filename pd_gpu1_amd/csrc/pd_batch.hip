@@ -305,9 +305,9 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         B_TRY(hipMemsetAsync(b->g_bar, 0, sizeof(unsigned) * groups, b->stream));
         B_TRY(hipMemsetAsync(b->g_abort, 0, sizeof(int), b->stream));
     }
-    hipEvent_t e0, e1;
-    B_TRY(hipEventCreate(&e0));
-    B_TRY(hipEventCreate(&e1));
+    HipEvents<2> ev;
+    B_TRY(ev.create());
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     B_TRY(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
     B_TRY(hipEventRecord(e0, b->stream));
     B_TRY(launch_batch_adr(b->stream, groups * W, block, b->group, b->view(), du, tol, max_iters, b->queue,
@@ -316,8 +316,6 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     B_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
     B_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (W > 1) {
         int aborted = 0;
         B_TRY(hipMemcpy(&aborted, b->g_abort, sizeof(int), hipMemcpyDeviceToHost));

@@ -16,6 +16,25 @@ struct CF3 { const float *p[3]; };
 static inline CD3 cd3(const D3 &a) { return CD3{{a.p[0], a.p[1], a.p[2]}}; }
 static inline CF3 cf3(const F3 &a) { return CF3{{a.p[0], a.p[1], a.p[2]}}; }
 
+// Timing events that are destroyed on every return path.
+template <int N>
+struct HipEvents {
+    hipEvent_t e[N] = {};
+    hipError_t create()
+    {
+        for (auto &ev : e) {
+            const hipError_t r = hipEventCreate(&ev);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+    ~HipEvents()
+    {
+        for (auto ev : e)
+            if (ev) (void)hipEventDestroy(ev);
+    }
+};
+
 // Device scalars of one handle (one allocation, written by kernels).
 struct DevScalars {
     double c;          // ADR damping coefficient of the last iteration
