@@ -304,3 +304,41 @@ def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None):
             extrap.close()
             warm.close()
     return rows
+
+
+def main(argv=None):
+    """Evaluate the warm start on unseen plates:
+    python -m pd_gpu1_amd.warmstart --plates 128 --increments 6
+    prints one line per increment and a JSON summary (totals of iterations and
+    seconds for plain ADR, the warm start without and with the network)."""
+    import argparse
+    import json
+    import os
+
+    ap = argparse.ArgumentParser(description="warm-start evaluation on unseen plates")
+    ap.add_argument("--plates", type=int, default=128)
+    ap.add_argument("--first", type=int, default=0, help="index of the first plate of the seeded family")
+    ap.add_argument("--increments", type=int, default=6)
+    ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                                      "mpnn_warmstart.npy"))
+    a = ap.parse_args(argv)
+    specs = eval_specs(a.first + a.plates)[a.first:]
+    with Mpnn() as net:
+        net.set_params(np.load(a.weights))
+        rows = np.array(evaluate(net, specs, a.increments, log=lambda m: print(m, flush=True)), dtype=float)
+    first, later = rows[:, 1] == 0, rows[:, 1] >= 1
+    tot = lambda sel, c: float(rows[sel, c].sum())
+    every = rows[:, 1] >= 0
+    out = {"plates": a.plates, "first": a.first, "sides": [int(s[0]) for s in specs], "increments": a.increments,
+           "iterations": {"plain": tot(every, 2), "warm_no_network": tot(every, 3), "warm_network": tot(every, 4)},
+           "seconds": {"plain": tot(every, 5), "warm_no_network": tot(every, 6), "warm_network": tot(every, 7)},
+           "first_increment_iterations": {"plain": tot(first, 2), "uniform_stretch": tot(first, 3),
+                                          "network": tot(first, 4)},
+           "later_increment_iterations": {"plain": tot(later, 2), "warm": tot(later, 3)},
+           "network_used": int(rows[:, 10].sum()), "network_fell_back": int(rows[first, 9].sum())}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    main()
