@@ -242,6 +242,12 @@ int pd_batch_get_parts(pd_batch *b, int *parts_used);
  * pd_batch_increment returns PD_E_HIP with the reason in
  * pd_batch_last_error (no reference counterpart). */
 int pd_batch_set_barrier_timeout(pd_batch *b, double seconds);
+/* The automatic workgroups per specimen from the specimens' bonds nb[],
+ * particles n[] and retired flags (nullable), for a device that holds
+ * `capacity` workgroups and at most wmax per specimen (no reference
+ * counterpart; host arithmetic only, no device is touched). */
+int pd_batch_choose_parts(const int64_t *nb, const int32_t *n, const int32_t *retired, int nspec, int capacity,
+                          int wmax);
 /* Warm start of the batched increments (no reference counterpart): 0 = plain
  * ADR from the previous solution with the strips moved (default), 1 = the
  * free particles also move by the previous increment, which is exact while

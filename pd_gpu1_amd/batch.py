@@ -26,6 +26,18 @@ def uniform01(state):
     return (v >> 11) * (1.0 / 9007199254740992.0), state
 
 
+def choose_parts(nb, n, capacity, wmax=16, retired=None):
+    """The automatic workgroups per specimen (pd_batch_choose_parts) for
+    specimens of nb bonds and n particles on a device that holds `capacity`
+    workgroups."""
+    nb = np.ascontiguousarray(nb, dtype=np.int64)
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    ret = None if retired is None else np.ascontiguousarray(retired, dtype=np.int32)
+    return int(lib().pd_batch_choose_parts(nb.ctypes.data_as(C.POINTER(C.c_int64)), n.ctypes.data_as(_lib.I32),
+                                           None if ret is None else ret.ctypes.data_as(_lib.I32), len(nb),
+                                           capacity, wmax))
+
+
 def plate_family(count, seed_base=0x5EED0003, n_min=40, n_max=200, dx=0.5e-3):
     """Seeded specimens of the batched family: side n ~ U{n_min..n_max}
     particles, crack length U(0.1, 0.5) * n * dx, angle U(-30, 30) degrees.

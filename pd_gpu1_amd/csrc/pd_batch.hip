@@ -254,24 +254,16 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     int W = b->parts;
     const int wmax = std::min(kMaxParts, block / 64);
     if (W == 0) {
-        // A specimen's work is its bonds times its ADR iterations, which grow
-        // with the plate's side (sqrt of its particles).  The largest
-        // specimen bounds the wall time from below by its work over W
-        // workgroups, the whole batch by all work over the device's
-        // workgroups: take the smallest power of two W that brings the first
-        // bound down to the second.
         std::vector<int32_t> ret((size_t)nspec, 0);
         if (b->retire)
             B_TRY(hipMemcpy(ret.data(), b->d_retired, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
-        double total = 0.0, largest = 0.0;
-        for (int q = 0; q < nspec; ++q)
-            if (!ret[q]) {
-                const double work = (double)b->spec[q].nb * std::sqrt((double)b->spec[q].n);
-                total += work;
-                largest = std::max(largest, work);
-            }
-        W = 1;
-        while (2 * W <= wmax && largest * capacity > total * W) W *= 2;
+        std::vector<int64_t> nb((size_t)nspec);
+        std::vector<int32_t> np((size_t)nspec);
+        for (int q = 0; q < nspec; ++q) {
+            nb[q] = b->spec[q].nb;
+            np[q] = b->spec[q].n;
+        }
+        W = pd_batch_choose_parts(nb.data(), np.data(), ret.data(), nspec, capacity, wmax);
     } else if (W > wmax) {
         b->err = "workgroups per specimen exceed block/64 (the damping tree's waves)";
         return PD_E_INVALID;
@@ -365,6 +357,28 @@ int pd_batch_set_parts(pd_batch *b, int parts)
     }
     b->parts = parts;
     return PD_OK;
+}
+
+int pd_batch_choose_parts(const int64_t *nb, const int32_t *n, const int32_t *retired, int nspec, int capacity,
+                          int wmax)
+{
+    // A specimen's work is its bonds times its ADR iterations, which grow
+    // with the plate's side (sqrt of its particles).  The largest specimen
+    // bounds the wall time from below by its work over W workgroups, the
+    // whole batch by all work over the device's workgroups: the smallest
+    // power of two W (at most wmax) that brings the first bound down to the
+    // second.  Host arithmetic only.
+    if (!nb || !n || nspec <= 0 || capacity <= 0 || wmax <= 0) return 1;
+    double total = 0.0, largest = 0.0;
+    for (int q = 0; q < nspec; ++q)
+        if (!(retired && retired[q])) {
+            const double work = (double)nb[q] * std::sqrt((double)n[q]);
+            total += work;
+            largest = std::max(largest, work);
+        }
+    int W = 1;
+    while (2 * W <= wmax && largest * capacity > total * W) W *= 2;
+    return W;
 }
 
 int pd_batch_set_barrier_timeout(pd_batch *b, double seconds)

@@ -57,3 +57,28 @@ def test_kw_crack_angles_recover_synthetic_cracks():
     inward = ob.crack_angle_from_tip(rad - t * np.sin(0.5), -t * np.cos(0.5), rad, 0.0, 1.0)
     assert abs(inward + np.degrees(0.5)) < 1e-6
     assert np.isnan(ob.crack_angle_from_tip([], [], 0.0, 0.0, 1.0))
+
+
+def test_batch_choose_parts_rule():
+    # pd_batch_choose_parts: the smallest power of two W (<= wmax) with
+    # largest work * capacity <= total work * W, work = bonds * sqrt(particles);
+    # host arithmetic only, so it runs without a device
+    from pd_gpu1_amd.batch import choose_parts
+    n = np.full(256, 10000)
+    nb = np.full(256, 280000)
+    assert choose_parts(nb, n, 256) == 1                 # one specimen per workgroup fills the device
+    assert choose_parts(nb[:64], n[:64], 256) == 4       # 64 equal specimens: 4 workgroups each
+    assert choose_parts(nb[:1], n[:1], 256) == 16        # one specimen: capped at wmax
+    assert choose_parts(nb[:1], n[:1], 256, wmax=4) == 4
+    # one specimen with 8x the work of each of 255 others (2x the bonds, 16x
+    # the particles): 8 * 256 <= 263 * 8 first at W = 8 (bonds alone would give 2)
+    nb2, n2 = nb.copy(), n.copy()
+    nb2[0] *= 2
+    n2[0] *= 16
+    assert choose_parts(nb2, n2, 256) == 8
+    # retired specimens do not count: all but one retired -> wmax
+    ret = np.ones(64, dtype=np.int32)
+    ret[5] = 0
+    assert choose_parts(nb[:64], n[:64], 256, retired=ret) == 16
+    assert choose_parts(nb[:64], n[:64], 256, retired=np.zeros(64, np.int32)) == 4
+    assert choose_parts(nb[:0], n[:0], 256) == 1         # nothing to run
