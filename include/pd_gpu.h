@@ -233,7 +233,11 @@ int pd_batch_set_group(pd_batch *b, int group);
  * then no longer sets the wall time alone).  The workgroups of a specimen
  * meet at barriers in global memory and are all resident (the grid is
  * capped); the damping sums keep the order of one workgroup, so results are
- * identical for every choice.  pd_batch_get_parts reports the value the last
+ * identical for every choice.  Other kernels running on the device at the
+ * same time (another batch's increment on another stream) can keep a group's
+ * workgroups from all being resident: the barrier timeout then ends the
+ * increment with an error, so run such increments one at a time per device
+ * or with parts = 1.  pd_batch_get_parts reports the value the last
  * increment ran with. */
 int pd_batch_set_parts(pd_batch *b, int parts);
 int pd_batch_get_parts(pd_batch *b, int *parts_used);
