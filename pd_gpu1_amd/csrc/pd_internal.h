@@ -47,8 +47,7 @@ struct DevScalars {
     int done;          // ADR stop flag (device-side termination)
     int iters;         // ADR iterations executed since the flag was cleared
     int capacity;      // neighbour-capacity overflow flag
-    unsigned ticket;   // last-block ticket of the fused reduction
-    unsigned pad[3];
+    unsigned pad[4];
 };
 
 // Bond coefficients recomputed from |xi| in the mixed kernels (see geom_coef
