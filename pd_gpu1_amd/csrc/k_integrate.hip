@@ -59,7 +59,7 @@ __device__ __forceinline__ void block_canon(Vec<NC> &p, double (*sh)[16])
 }
 
 // Cross-block reduction in two launches: every block writes its canonical
-// partial, and a one-thread finishing kernel adds the G partials with the
+// partial, and a one-workgroup finishing kernel adds the G partials with the
 // fixed tree (offsets 1, 2, 4, ... over the blocks).  Measured faster than
 // finishing in the last block of the same launch (fences and a ticket):
 // an ADR iteration on a 315 x 315 plate 96 -> 62 us.
@@ -71,13 +71,26 @@ __device__ __forceinline__ void block_partial(Vec<NC> &p, double *partials, doub
         for (int c = 0; c < NC; ++c) partials[blockIdx.x * NC + c] = p.v[c];
 }
 
+// Called by every thread of a one-workgroup launch; thread 0 returns true
+// with the totals.  The partials are loaded into LDS together (a tree over
+// them in global memory would wait for one load per step).
+constexpr int kFinishLds = 1024;  // doubles of partials the LDS path holds
+
 template <int NC>
-__device__ __forceinline__ void finish_tree(int G, double *partials, Vec<NC> &tot)
+__device__ __forceinline__ bool finish_tree(int G, double *partials, Vec<NC> &tot)
 {
+    __shared__ double q[kFinishLds];
+    const bool in_lds = G * NC <= kFinishLds;
+    if (in_lds)
+        for (int k = threadIdx.x; k < G * NC; k += blockDim.x) q[k] = partials[k];
+    __syncthreads();
+    if (threadIdx.x != 0) return false;
+    double *v = in_lds ? q : partials;
     for (int off = 1; off < G; off <<= 1)
         for (int b = 0; b < G; b += 2 * off)
-            for (int c = 0; c < NC; ++c) partials[b * NC + c] = partials[b * NC + c] + partials[(b + off) * NC + c];
-    for (int c = 0; c < NC; ++c) tot.v[c] = partials[c];
+            for (int c = 0; c < NC; ++c) v[b * NC + c] = v[b * NC + c] + v[(b + off) * NC + c];
+    for (int c = 0; c < NC; ++c) tot.v[c] = v[c];
+    return true;
 }
 
 template <int DIM>
@@ -301,13 +314,12 @@ __global__ __launch_bounds__(1024) void k_adr_reduce(int N, const double *__rest
 }
 
 // The damping coefficient, the residual and the stop flag from the G block
-// partials of k_adr_reduce (one thread).
+// partials of k_adr_reduce (one workgroup; thread 0 writes).
 __global__ void k_adr_finish(int G, double *partials, DevScalars *S, double tol)
 {
     if (S->done) return;
     Vec<4> tot;
-    finish_tree<4>(G, partials, tot);
-    {
+    if (finish_tree<4>(G, partials, tot)) {
         double c = adr_c(tot.v[0], tot.v[1]);
         double rnum = tot.v[2], rden = tot.v[3];
         double r = rden > 0.0 ? sqrt(rnum) / sqrt(rden) : (rnum > 0.0 ? INFINITY : 0.0);
@@ -446,8 +458,7 @@ __global__ __launch_bounds__(1024) void k_contact_sum(int N, const double *__res
 __global__ void k_contact_finish(int G, double *partials, pd_cyl cyl, double dt, DevScalars *S)
 {
     Vec<3> tot;
-    finish_tree<3>(G, partials, tot);
-    {
+    if (finish_tree<3>(G, partials, tot)) {
         double *cs = S->cyl;  // vel[0..2] disp[3..5] acc[6..8] force[9..11]
         for (int d = 0; d < 3; ++d) {
             cs[9 + d] = tot.v[d];
@@ -473,8 +484,7 @@ __global__ __launch_bounds__(1024) void k_field_sum(const double *__restrict__ v
 __global__ void k_field_finish(int G, double *partials, DevScalars *S)
 {
     Vec<1> tot;
-    finish_tree<1>(G, partials, tot);
-    S->sums[7] = tot.v[0];
+    if (finish_tree<1>(G, partials, tot)) S->sums[7] = tot.v[0];
 }
 
 __global__ void k_strip_displace(int N_int, int n_top, int n_bot, double dtop, double dbot, double *uy)
@@ -579,7 +589,7 @@ hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, in
     k_adr_reduce<<<blocks, threads, 0, s>>>(N, terms, partials, S, tol);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_adr_finish<<<1, 1, 0, s>>>(blocks, partials, S, tol);
+    k_adr_finish<<<1, 64, 0, s>>>(blocks, partials, S, tol);
     return hipGetLastError();
 }
 
@@ -608,7 +618,7 @@ hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const d
     k_contact_sum<<<blocks, threads, 0, s>>>(N, terms, cyl, dt, partials, S);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_contact_finish<<<1, 1, 0, s>>>(blocks, partials, cyl, dt, S);
+    k_contact_finish<<<1, 64, 0, s>>>(blocks, partials, cyl, dt, S);
     return hipGetLastError();
 }
 
@@ -618,7 +628,7 @@ hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double
     k_field_sum<<<blocks, threads, 0, s>>>(v, n, partials, S);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_field_finish<<<1, 1, 0, s>>>(blocks, partials, S);
+    k_field_finish<<<1, 64, 0, s>>>(blocks, partials, S);
     return hipGetLastError();
 }
 
