@@ -6,7 +6,7 @@
 // a power of two): lane t sums elements t, t+T, t+2T, ... in sequence; the T
 // partials are then combined pairwise at offsets 1, 2, 4, ... (partial[t] +=
 // partial[t+off] for t % (2*off) == 0) -- xor/down shuffles inside a wave64,
-// LDS across waves, a last-block pass across blocks.  The result is
+// LDS across waves, a one-workgroup finishing launch across blocks.  The result is
 // deterministic and independent of scheduling; the FP64 oracle emulates the
 // same order, so ADR damping coefficients and trajectories match it bit for
 // bit.  This replaces reduce_cp + sum() (Reduce.cu:11-65), which copies each
@@ -424,8 +424,8 @@ __global__ void k_contact_terms(int N, const double *__restrict__ dmg, CD3 x, D3
     terms[2 * (size_t)N + i] = r2;
 }
 
-// The reaction on the cylinder in canonical order, then the cylinder's second
-// half step (Solve_Kalthoff_Winkler.cu:255-265) in the last block.
+// The reaction on the cylinder in canonical order (block partials here, the
+// tree across blocks and the cylinder's second half step in k_contact_finish).
 __global__ __launch_bounds__(1024) void k_contact_sum(int N, const double *__restrict__ terms, pd_cyl cyl,
                                                       double dt, double *partials, DevScalars *S)
 {
