@@ -190,9 +190,15 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         int it = 0;
         double r = 0.0;
         for (;;) {
-            // PMB force, every row (Force_Bond.cu:169)
+            // PMB force, every row (Force_Bond.cu:169).  A part computes the
+            // rows of the particles on its own damping-tree lanes (i mod BT
+            // in [part*L, part*L + L)), so the damping sums below need only
+            // a workgroup barrier; own_row(q) ascends with q.
+            const auto own_row = [&](int q) { return part * L + q % L + (q / L) * BT; };
             if (G == 1) {
-                for (int i = gt; i < N; i += GT) {
+                for (int q = tid;; q += BT) {
+                    const int i = own_row(q);
+                    if (i >= N) break;
                     const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
                     double fx = 0.0, fy = 0.0;
                     for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
@@ -207,7 +213,9 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                 }
             } else {
                 const int lane = tid & (G - 1);
-                for (int i = gt / G; i < N; i += GT / G) {  // the lane group takes the row
+                for (int q = tid / G;; q += BT / G) {  // the lane group takes the row
+                    const int i = own_row(q);
+                    if (i >= N) break;
                     const double x0 = X[i], x1 = Y[i], u0 = UX[i], u1 = UY[i];
                     double fx = 0.0, fy = 0.0;
                     const int32_t kb = rp[i], ke = rp[i + 1];
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                     }
                 }
             }
-            PD_GROUP_SYNC();
+            __syncthreads();  // this part's lanes read only rows this workgroup computed
             // damping and residual terms, lane t of the T = BT lane tree owns
             // particles t, t+BT, ...; this part holds lanes part*L ... part*L+L-1
             double p[4] = {0.0, 0.0, 0.0, 0.0};
