@@ -95,7 +95,7 @@ __device__ __forceinline__ bool group_sync(const BatchGroups &gr, int grp, unsig
     __syncthreads();
     if (threadIdx.x == 0) {
         arrivals += (unsigned)gr.parts;
-        __threadfence();  // this workgroup's writes before its arrival
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's writes before its arrival
         atomicAdd(&gr.bar[grp], 1u);
         const uint64_t t0 = wall_clock64();
         int ok = 1;
@@ -108,7 +108,7 @@ __device__ __forceinline__ bool group_sync(const BatchGroups &gr, int grp, unsig
             }
             __builtin_amdgcn_s_sleep(2);
         }
-        __threadfence();  // the group's writes before this workgroup's reads
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the group's writes before this workgroup's reads
         ok_sh = ok;
     }
     __syncthreads();
