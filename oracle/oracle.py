@@ -39,11 +39,15 @@ _lib = None
 
 
 def lib():
+    """The loaded oracle library.  PD_ORACLE_LIB names another build of the
+    same source (the tests' planted-mistake copies); otherwise liboracle.so
+    next to this file, built on first use."""
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        path = os.environ.get("PD_ORACLE_LIB", _LIB_PATH)
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         L.orc_cal_parameter.argtypes = [C.POINTER(OrcParams), C.c_int, C.c_int] + [C.c_double] * 10 + [C.c_double, C.c_int]
         L.orc_coord_plate.argtypes = [C.c_int, C.c_int, C.c_double, D, D]
         L.orc_coord_plate_strips.argtypes = [C.c_int, C.c_int, C.c_double, D, D]
