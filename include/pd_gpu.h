@@ -242,9 +242,12 @@ int pd_batch_set_group(pd_batch *b, int group);
 int pd_batch_set_parts(pd_batch *b, int parts);
 int pd_batch_get_parts(pd_batch *b, int *parts_used);
 /* Longest wait (seconds, default 20) of a workgroup at a group barrier; a
- * longer wait abandons the increment: every workgroup leaves the kernel and
+ * longer wait abandons the increment: every workgroup leaves the kernel,
+ * every specimen's state (displacements, velocities, forces, fail bits,
+ * retired flags) is put back to what it was before the increment, and
  * pd_batch_increment returns PD_E_HIP with the reason in
- * pd_batch_last_error (no reference counterpart). */
+ * pd_batch_last_error.  The batch stays usable: the same increment can be
+ * run again, e.g. with parts = 1 (no reference counterpart). */
 int pd_batch_set_barrier_timeout(pd_batch *b, double seconds);
 /* The automatic workgroups per specimen from the specimens' bonds nb[],
  * particles n[] and retired flags (nullable), for a device that holds

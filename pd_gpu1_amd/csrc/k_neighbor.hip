@@ -198,6 +198,8 @@ hipError_t launch_neigh_fill(hipStream_t s, int dim, int N, CD3 x, const double 
     return hipGetLastError();
 }
 
+constexpr int kMaxRowSort = 256;  // longest row k_row_sort handles
+
 hipError_t launch_row_sort(hipStream_t s, int N, const int32_t *row_ptr, int32_t *col, int max_row)
 {
     int blocks = (N + 3) / 4;
@@ -282,6 +284,11 @@ int build_neighbors_impl(pd_handle *h)
     if (max_row > h->P.max_neighbors) {
         h->err = "neighbour count " + std::to_string(max_row) + " exceeds max_neighbors " +
                  std::to_string(h->P.max_neighbors);
+        return PD_E_CAPACITY;
+    }
+    if (max_row > kMaxRowSort) {
+        h->err = "neighbour count " + std::to_string(max_row) + " exceeds the " + std::to_string(kMaxRowSort) +
+                 " neighbours per row the CSR row sort holds";
         return PD_E_CAPACITY;
     }
     if ((int64_t)max_row * N >= ((int64_t)1 << 31)) {

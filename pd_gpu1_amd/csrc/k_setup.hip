@@ -182,17 +182,26 @@ __global__ void k_crack_kw(int N_int, const int32_t *__restrict__ rp, const int3
 }
 
 __global__ void k_lps_weights(int N, const int32_t *__restrict__ rp, const double *__restrict__ idist,
-                              const double *__restrict__ fac, double delta, double vol, double *w, double *m)
+                              const double *__restrict__ fac, double delta, double vol, int w_given, double *w,
+                              double *m)
 {
     // Force_State.cu:309-321
+    // w_given: w[] already holds the influence weights (computed on the host
+    // with the C library's exp, so they equal the FP64 CPU algorithm's bit for
+    // bit); only m is formed here, in the same order.
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double mi = 0.0;
     const double d2 = delta * delta;
     for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
         double id = idist[k];
-        double wk = exp(-(id * id) / d2);
-        w[k] = wk;
+        double wk;
+        if (w_given) {
+            wk = w[k];
+        } else {
+            wk = exp(-(id * id) / d2);
+            w[k] = wk;
+        }
         mi += wk * (id * id) * vol * fac[k];
     }
     m[i] = mi;
@@ -316,9 +325,10 @@ hipError_t launch_crack_kw(hipStream_t s, int N_int, const int32_t *rp, const in
 }
 
 hipError_t launch_lps_weights(hipStream_t s, int N, const int32_t *rp, const double *idist,
-                              const double *fac, double delta, double vol, double *w, double *m)
+                              const double *fac, double delta, double vol, int w_given, double *w, double *m)
 {
-    k_lps_weights<<<nblk(N, 256), 256, 0, s>>>(N, rp, idist, fac, delta, vol, w, m);
+    if (N <= 0) return hipSuccess;
+    k_lps_weights<<<nblk(N, 256), 256, 0, s>>>(N, rp, idist, fac, delta, vol, w_given, w, m);
     return hipGetLastError();
 }
 

@@ -9,7 +9,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -481,9 +483,16 @@ int read_scalars(pd_handle *h, DevScalars *out)
     return PD_OK;
 }
 
+// Destroys the cached step graph.  Launches of it may still be queued on the
+// handle's stream (pd_run_verlet's odd fused step drops it right after
+// replaying it; an error return can leave replays queued), so the stream is
+// drained first: the executable graph is never destroyed while in flight.
 void drop_graph(pd_handle *h)
 {
-    if (h->step_graph) (void)hipGraphExecDestroy(h->step_graph);
+    if (h->step_graph) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipGraphExecDestroy(h->step_graph);
+    }
     h->step_graph = nullptr;
     h->step_graph_kind = -1;
 }
@@ -917,12 +926,46 @@ int pd_set_notches_kw(pd_handle *h, double rad)
     return PD_OK;
 }
 
+// Influence weights w = exp(-xi^2/delta^2) (Force_State.cu:317) on the host:
+// the C library's exp is the one the FP64 CPU algorithm calls, so w, and with
+// it every FP64 LPS field, is bit-identical to that algorithm's.  The bond
+// lengths go down and the weights up in chunks, each chunk split over host
+// threads.
+static hipError_t host_lps_weights(hipStream_t s, int64_t nb, const double *idist, double delta, double *w)
+{
+    const int64_t chunk = int64_t(1) << 25;
+    std::vector<double> buf((size_t)std::min<int64_t>(nb, chunk));
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
+    const double d2 = delta * delta;
+    for (int64_t k0 = 0; k0 < nb; k0 += chunk) {
+        const int64_t n = std::min(chunk, nb - k0);
+        hipError_t e = hipMemcpyAsync(buf.data(), idist + k0, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        const unsigned use = n < (int64_t)1 << 16 ? 1 : nt;
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < use; ++t)
+            pool.emplace_back([&, t] {
+                const int64_t a = n * t / use, b = n * (t + 1) / use;
+                for (int64_t k = a; k < b; ++k) buf[k] = std::exp(-(buf[k] * buf[k]) / d2);
+            });
+        for (auto &th : pool) th.join();
+        e = hipMemcpyAsync(w + k0, buf.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 int pd_lps_weights(pd_handle *h)
 {
     if (!h) return PD_E_INVALID;
     REQUIRE(h->col && h->P.model == PD_MODEL_LPS, "LPS problem with bonds required");
     LAUNCH(dalloc(&h->w, h->Nb));
-    LAUNCH(launch_lps_weights(h->stream, h->N, h->row_ptr, h->idist, h->fac, h->P.delta, h->P.vol, h->w, h->m));
+    LAUNCH(host_lps_weights(h->stream, h->Nb, h->idist, h->P.delta, h->w));
+    LAUNCH(launch_lps_weights(h->stream, h->N, h->row_ptr, h->idist, h->fac, h->P.delta, h->P.vol, 1, h->w,
+                              h->m));
     h->have_weights = true;
     h->coef_dirty = true;
     return PD_OK;

@@ -41,6 +41,10 @@ struct pd_batch {
     int *g_spec = nullptr;        // [groups_cap] specimen of each group
     double *g_sum = nullptr;      // [groups_cap][kMaxParts][4] damping subtree roots
     int *g_abort = nullptr;       // a group barrier timed out
+    double *snap = nullptr;       // state before a multi-workgroup increment: 14 particle fields,
+    double2 *snap_pos = nullptr;  // positions, fail words and retired flags, restored when
+    uint32_t *snap_fail = nullptr;  // the increment is abandoned at a barrier timeout
+    int32_t *snap_ret = nullptr;
     double barrier_timeout = 20.0;  // seconds a workgroup waits at a group barrier
     int warm = 0;                 // 1: increments start from u + the previous increment
     BatchView view() const
@@ -76,7 +80,8 @@ struct pd_batch {
 static void batch_free(pd_batch *b)
 {
     void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->fail, b->queue, b->d_iters,
-                    b->d_resid, b->d_retired, b->g_bar, b->g_spec, b->g_sum, b->g_abort};
+                    b->d_resid, b->d_retired, b->g_bar, b->g_spec, b->g_sum, b->g_abort, b->snap, b->snap_pos,
+                    b->snap_fail, b->snap_ret};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -297,6 +302,26 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         B_TRY(hipMemsetAsync(b->g_bar, 0, sizeof(unsigned) * groups, b->stream));
         B_TRY(hipMemsetAsync(b->g_abort, 0, sizeof(int), b->stream));
     }
+    // With several workgroups per specimen a barrier timeout abandons the
+    // increment part-way (strips moved, some specimens relaxed, others not):
+    // keep the state it starts from so that it can be put back.
+    const size_t nfield = (size_t)b->n_total * 14, nwords = (size_t)(b->nb_total + 31) / 32;
+    const size_t nspec_sz = b->spec.size();
+    if (W > 1) {
+        if (!b->snap) {
+            B_TRY(hipMalloc((void **)&b->snap, sizeof(double) * nfield));
+            B_TRY(hipMalloc((void **)&b->snap_pos, sizeof(double2) * b->n_total));
+            B_TRY(hipMalloc((void **)&b->snap_fail, sizeof(uint32_t) * nwords));
+            B_TRY(hipMalloc((void **)&b->snap_ret, sizeof(int32_t) * nspec_sz));
+        }
+        double *u0 = b->buf + (size_t)2 * b->n_total;  // fields 2..15 are contiguous
+        B_TRY(hipMemcpyAsync(b->snap, u0, sizeof(double) * nfield, hipMemcpyDeviceToDevice, b->stream));
+        B_TRY(hipMemcpyAsync(b->snap_pos, b->pos, sizeof(double2) * b->n_total, hipMemcpyDeviceToDevice,
+                             b->stream));
+        B_TRY(hipMemcpyAsync(b->snap_fail, b->fail, sizeof(uint32_t) * nwords, hipMemcpyDeviceToDevice, b->stream));
+        B_TRY(hipMemcpyAsync(b->snap_ret, b->d_retired, sizeof(int32_t) * nspec_sz, hipMemcpyDeviceToDevice,
+                             b->stream));
+    }
     HipEvents<2> ev;
     B_TRY(ev.create());
     hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
@@ -312,7 +337,17 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         int aborted = 0;
         B_TRY(hipMemcpy(&aborted, b->g_abort, sizeof(int), hipMemcpyDeviceToHost));
         if (aborted) {
-            b->err = "a workgroup group barrier timed out (the group's workgroups were not all resident)";
+            double *u0 = b->buf + (size_t)2 * b->n_total;
+            B_TRY(hipMemcpyAsync(u0, b->snap, sizeof(double) * nfield, hipMemcpyDeviceToDevice, b->stream));
+            B_TRY(hipMemcpyAsync(b->pos, b->snap_pos, sizeof(double2) * b->n_total, hipMemcpyDeviceToDevice,
+                                 b->stream));
+            B_TRY(hipMemcpyAsync(b->fail, b->snap_fail, sizeof(uint32_t) * nwords, hipMemcpyDeviceToDevice,
+                                 b->stream));
+            B_TRY(hipMemcpyAsync(b->d_retired, b->snap_ret, sizeof(int32_t) * nspec_sz, hipMemcpyDeviceToDevice,
+                                 b->stream));
+            B_TRY(hipStreamSynchronize(b->stream));
+            b->err = "a workgroup group barrier timed out (the group's workgroups were not all resident); the "
+                     "increment was abandoned and every specimen restored to its state before it";
             return PD_E_HIP;
         }
     }

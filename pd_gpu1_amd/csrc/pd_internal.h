@@ -200,7 +200,7 @@ hipError_t launch_crack_2d(hipStream_t s, int N_int, const int32_t *rp, const in
 hipError_t launch_crack_kw(hipStream_t s, int N_int, const int32_t *rp, const int32_t *col, CD3 x,
                            double rad, uint32_t *fail);
 hipError_t launch_lps_weights(hipStream_t s, int N, const int32_t *rp, const double *idist,
-                              const double *fac, double delta, double vol, double *w, double *m);
+                              const double *fac, double delta, double vol, int w_given, double *w, double *m);
 hipError_t launch_pmb_coef(hipStream_t s, int64_t nb, const double *fac, const double *scr, double vol,
                            double bc, const double *idist, double *kappa, float2 *bnd);
 hipError_t launch_lps_coef(hipStream_t s, int64_t nb, const double *w, const double *fac, double vol,

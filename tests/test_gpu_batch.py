@@ -116,14 +116,31 @@ def test_batch_retire_skips_only_unconverged_specimens():
 def test_batch_group_barrier_timeout_is_reported():
     # a one-tick bound on the group barriers' wait: every workgroup leaves the
     # kernel at its next wait and the increment reports the timeout
+    # kernel at its next wait and the increment reports the timeout; the
+    # specimens are restored to their state before the increment, so the
+    # same handle then runs the increment as a fresh batch does
     p = PB.plate_params()
     specs = PB.plate_family(8, n_min=40, n_max=80)
+    with PB.Batch(p, specs) as ref:
+        ref.set_parts(1)
+        ref.increment(1.0e-6, 1e-6, 200000, block=1024)
+        its_ref, res_ref, _ = ref.increment(1.0e-6, 1e-6, 200000, block=1024)
+        u_ref = [ref.get(s) for s in range(len(specs))]
+        fail_ref = [ref.csr(s)[2] for s in range(len(specs))]
     with PB.Batch(p, specs) as b:
+        b.set_parts(1)
+        b.increment(1.0e-6, 1e-6, 200000, block=1024)
+        before = [(b.get(s), b.get(s, "VHO"), b.get(s, "FO"), b.csr(s)[2]) for s in range(len(specs))]
         b.set_parts(4)
         b.set_barrier_timeout(1e-9)
-        with pytest.raises(PdError, match="timed out"):
+        with pytest.raises(PdError, match="restored"):
             b.increment(1.0e-6, 1e-6, 2000, block=1024)
-    with PB.Batch(p, specs) as b:  # the default bound: the same batch converges
-        b.set_parts(4)
+        for s in range(len(specs)):
+            after = (b.get(s), b.get(s, "VHO"), b.get(s, "FO"), b.csr(s)[2])
+            assert all(np.array_equal(x, y) for x, y in zip(before[s], after)), s
+        b.set_barrier_timeout(20.0)  # the default bound: the same increment converges
         its, res, _ = b.increment(1.0e-6, 1e-6, 200000, block=1024)
         assert b.parts_used() == 4 and np.all(res <= 1e-6)
+        assert np.array_equal(its, its_ref) and np.array_equal(res, res_ref)
+        for s in range(len(specs)):
+            assert np.array_equal(b.get(s), u_ref[s]) and np.array_equal(b.csr(s)[2], fail_ref[s]), s
