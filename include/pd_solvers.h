@@ -3,7 +3,9 @@
  *
  * Each mirrors one solver of XiaoMmingW/PD_Gpu1 (scenario constants, stage
  * order, disp.txt output) and runs it on GPU `gpu`.  They return a pd_gpu.h
- * status instead of calling exit(1).
+ * status instead of calling exit(1).  checkpoint_path, when not NULL, receives
+ * the final state in the binary format of pd_checkpoint_write (no reference
+ * counterpart: disp.txt keeps six significant digits).
  */
 #ifndef PD_SOLVERS_H
 #define PD_SOLVERS_H
@@ -23,12 +25,13 @@ typedef struct Dselect {
 
 /* solve_plate_tensile(Dselect&, real dx) -- Solve_Plate_Tensile.cuh:17.
  * dx <= 1e-8 selects the reference default 0.01 m. */
-int solve_plate_tensile(Dselect *d, double dx, int gpu, const char *out_path);
+int solve_plate_tensile(Dselect *d, double dx, int gpu, const char *out_path, const char *checkpoint_path);
 /* solve_plate_crack_grow(Dselect&, real dx) -- Solve_Plate_Crack_Grow.cuh:18. */
-int solve_plate_crack_grow(Dselect *d, double dx, int gpu, const char *out_path);
+int solve_plate_crack_grow(Dselect *d, double dx, int gpu, const char *out_path, const char *checkpoint_path);
 /* solve_kalthoff_winkler(Dselect&, real dx) -- Solve_Kalthoff_Winkler.cuh:18.
  * nt < 0 runs 1350 steps (the shipped nt = 0 runs none). */
-int solve_kalthoff_winkler(Dselect *d, double dx, int gpu, const char *out_path, int nt);
+int solve_kalthoff_winkler(Dselect *d, double dx, int gpu, const char *out_path, int nt,
+                           const char *checkpoint_path);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,9 @@
 // argument instead of cudaSetDevice(1) (main.cu:16); the Kalthoff-Winkler loop
 // runs 1350 steps instead of the shipped nt = 0 (Solve_Kalthoff_Winkler.cu:280);
 // the CPU path (device 1) is not part of this engine and is reported as an
-// error; failures return a status instead of exit(1).
+// error; failures return a status instead of exit(1).  checkpoint_path
+// (nullable) receives the final state in the binary pd_checkpoint_write
+// format, for comparisons beyond disp.txt's six significant digits.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -59,7 +61,7 @@ int check_device(const Dselect &d)
 
 } // namespace
 
-extern "C" int solve_plate_tensile(Dselect *d, double dx, int gpu, const char *out_path)
+extern "C" int solve_plate_tensile(Dselect *d, double dx, int gpu, const char *out_path, const char *checkpoint_path)
 {
     // Solve_Plate_Tensile.cu:30-138
     int rc = check_device(*d);
@@ -87,11 +89,13 @@ extern "C" int solve_plate_tensile(Dselect *d, double dx, int gpu, const char *o
     r.ok(pd_damage(r.h)) && r.ok(pd_sync(r.h));
     if (r.rc) return r.rc;
     std::printf("time %f\n", (now_s() - t0) * 1e6);
+    if (checkpoint_path) r.ok(pd_checkpoint_write(r.h, checkpoint_path, nt));
     r.ok(pd_save_text(r.h, out_path ? out_path : "disp.txt", PD_LAYOUT_PLATE, nx, ny, 1, 0.0));
     return r.rc;
 }
 
-extern "C" int solve_plate_crack_grow(Dselect *d, double dx, int gpu, const char *out_path)
+extern "C" int solve_plate_crack_grow(Dselect *d, double dx, int gpu, const char *out_path,
+                                      const char *checkpoint_path)
 {
     // Solve_Plate_Crack_Grow.cu:30-171
     int rc = check_device(*d);
@@ -123,11 +127,13 @@ extern "C" int solve_plate_crack_grow(Dselect *d, double dx, int gpu, const char
     r.ok(pd_run_verlet(r.h, nt)) && r.ok(pd_damage(r.h)) && r.ok(pd_sync(r.h));
     if (r.rc) return r.rc;
     std::printf("time %f\n", (now_s() - t0) * 1e6);
+    if (checkpoint_path) r.ok(pd_checkpoint_write(r.h, checkpoint_path, nt));
     r.ok(pd_save_text(r.h, out_path ? out_path : "disp.txt", PD_LAYOUT_PLATE, nx, ny, 1, 0.0));
     return r.rc;
 }
 
-extern "C" int solve_kalthoff_winkler(Dselect *d, double dx, int gpu, const char *out_path, int nt)
+extern "C" int solve_kalthoff_winkler(Dselect *d, double dx, int gpu, const char *out_path, int nt,
+                                      const char *checkpoint_path)
 {
     // Solve_Kalthoff_Winkler.cu:268-441
     int rc = check_device(*d);
@@ -166,6 +172,7 @@ extern "C" int solve_kalthoff_winkler(Dselect *d, double dx, int gpu, const char
     r.ok(pd_run_verlet(r.h, nt)) && r.ok(pd_damage(r.h)) && r.ok(pd_sync(r.h));
     if (r.rc) return r.rc;
     std::printf("time %f\n", (now_s() - t0) * 1e6);
+    if (checkpoint_path) r.ok(pd_checkpoint_write(r.h, checkpoint_path, nt));
     r.ok(pd_save_text(r.h, out_path ? out_path : "disp.txt", PD_LAYOUT_KW, nx, ny, nz, cyl.rad));
     return r.rc;
 }

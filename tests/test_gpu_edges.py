@@ -151,17 +151,22 @@ def test_entry_points_reject_invalid_arguments():
     assert L.pd_batch_create(C.byref(p3), spec, 1, 0, C.byref(hb)) == _lib.PD_E_INVALID
 
 
-def test_rows_longer_than_the_row_sort_are_a_capacity_error():
-    # delta = 4.5 dx on a cubic lattice gives interior rows of 380 bonds: the
-    # count fits max_neighbors = 512 but not the 256-entry row sort
+def test_rows_longer_than_256_neighbours_are_a_capacity_error():
+    # delta = 4.5 dx on a cubic lattice gives interior rows of 380 bonds, more
+    # than the 256 a row may hold (max_neighbors is capped there: the CSR row
+    # sort holds 256 entries)
     p, _, _ = params(3, _lib.PD_MODEL_PMB, 1.0e-8, DX, length=0.012, height=0.012, width=0.012, emod=192e9,
                      pratio=0.25, dens=8000.0, s0=0.004, delta_factor=4.5)
-    p.max_neighbors = 512
+    p.max_neighbors = 257
+    with pytest.raises(pd.PdError) as ei:
+        pd.Problem(p, 0)
+    assert ei.value.code == _lib.PD_E_INVALID
+    p.max_neighbors = 256
     g = (np.arange(12) - 5.5) * DX
     x, y, z = (a.ravel().copy() for a in np.meshgrid(g, g, g, indexing="ij"))
     prob = pd.Problem(p, 0)
     prob.set_positions(x, y, z)
     with pytest.raises(pd.PdError) as ei:
         prob.build_neighbors()
-    assert ei.value.code == _lib.PD_E_CAPACITY and "row sort" in str(ei.value)
+    assert ei.value.code == _lib.PD_E_CAPACITY and "exceeds max_neighbors 256" in str(ei.value)
     prob.close()
