@@ -175,6 +175,19 @@ int pd_adr_solve(pd_handle *h, int ct0, int max_iters, double tol,
 int pd_strip_displace(pd_handle *h, double du_top, double du_bot);
 /* Reduce.cu:52-65 sum() test hook: fixed-tree sum of field component comp. */
 int pd_reduce_sum(pd_handle *h, int field, int comp, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the fixed-tree sum of pd_reduce_sum over n doubles at the
+ * device pointer `values`, with `lanes` lanes (a power of two, as
+ * pd_params.reduce_lanes; equal lanes give the bits pd_reduce_sum gives),
+ * written to the device double `out`, enqueued on `stream` (a hipStream_t,
+ * NULL = the null stream) on GPU `device`; nothing waits for it.  It
+ * allocates nothing: the block partials live in a 32 KiB device array the
+ * library keeps on each GPU until the process ends, so lanes above
+ * 4096 * 256 return PD_E_CAPACITY, and calls that overlap on one GPU (on two
+ * streams at once) race for that array.  PD_E_INVALID: null pointers, n < 0,
+ * lanes not a power of two >= 64, or a bad device. */
+int pd_reduce_sum_noalloc(int device, const double *values, int64_t n, int lanes, double *out, void *stream);
 
 /* ---- damage and fields (Base_Function.cu:149-217) ------------------------ */
 int pd_damage(pd_handle *h);

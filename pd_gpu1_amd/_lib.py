@@ -26,7 +26,7 @@ EXPORTS = (
     "pd_build_neighbors", "pd_num_bonds", "pd_get_csr", "pd_vol_corr", "pd_surface_correct",
     "pd_init_state", "pd_set_crack_2d", "pd_set_notches_kw", "pd_lps_weights", "pd_force",
     "pd_verlet_half1", "pd_verlet_half2", "pd_run_verlet", "pd_set_body_force_tensile",
-    "pd_adr_iterate", "pd_adr_solve", "pd_strip_displace", "pd_reduce_sum", "pd_damage",
+    "pd_adr_iterate", "pd_adr_solve", "pd_strip_displace", "pd_reduce_sum", "pd_reduce_sum_noalloc", "pd_damage",
     "pd_get_field", "pd_set_field", "pd_get_fail_bits", "pd_set_fail_bits", "pd_cylinder_init",
     "pd_cylinder_get", "pd_contact_step", "pd_save_text", "pd_checkpoint_write", "pd_checkpoint_read",
     "pd_last_force_ms", "pd_time_force_passes", "pd_set_group", "pd_set_pass_group", "pd_stream_copy", "pd_set_tile", "pd_set_lds", "pd_set_lds_block", "pd_set_lds_pass_block", "pd_lds_info", "pd_set_coef_mode", "pd_coef_info", "pd_set_fp64_group", "pd_set_fused_verlet", "pd_sync",
@@ -104,6 +104,7 @@ def lib():
             "pd_adr_solve": [H, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int), D],
             "pd_strip_displace": [H, C.c_double, C.c_double],
             "pd_reduce_sum": [H, C.c_int, C.c_int, D],
+            "pd_reduce_sum_noalloc": [C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p],
             "pd_damage": [H],
             "pd_get_field": [H, C.c_int, D],
             "pd_set_field": [H, C.c_int, D],

@@ -487,6 +487,13 @@ __global__ void k_field_finish(int G, double *partials, DevScalars *S)
     if (finish_tree<1>(G, partials, tot)) S->sums[7] = tot.v[0];
 }
 
+// The same finish, writing the total to a caller's device double.
+__global__ void k_field_finish_to(int G, double *partials, double *out)
+{
+    Vec<1> tot;
+    if (finish_tree<1>(G, partials, tot)) *out = tot.v[0];
+}
+
 __global__ void k_strip_displace(int N_int, int n_top, int n_bot, double dtop, double dbot, double *uy)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -629,6 +636,16 @@ hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_field_finish<<<1, 64, 0, s>>>(blocks, partials, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_sum_to(hipStream_t s, int blocks, int threads, const double *v, int64_t n, double *partials,
+                               double *out)
+{
+    k_field_sum<<<blocks, threads, 0, s>>>(v, n, partials, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_field_finish_to<<<1, 64, 0, s>>>(blocks, partials, out);
     return hipGetLastError();
 }
 

@@ -278,6 +278,8 @@ hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const d
 hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done);
 hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double *v, int64_t n,
                             double *partials, DevScalars *S);
+hipError_t launch_field_sum_to(hipStream_t s, int blocks, int threads, const double *v, int64_t n, double *partials,
+                               double *out);
 hipError_t launch_strip_displace(hipStream_t s, int N_int, int n_top, int n_bot, double dtop,
                                  double dbot, double *uy);
 hipError_t launch_f32_to_f64(hipStream_t s, const float *a, double *b, int64_t n);
