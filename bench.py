@@ -57,21 +57,36 @@ def kernel_stream_bytes(nb, n, dim=3, lds=True, recomputed=True):
     return per_bond * nb + (64 + 4 + 4 * dim) * n
 
 
+def batch_iteration_bytes(nb, n):
+    """Algorithmic bytes of one ADR iteration of one 2D specimen in the
+    batched kernel (FP64): the PMB force as counted for FP64 PMB, 20.125 B per
+    bond (col 4, |xi| 8, kappa 8, fail bit 1/8) and 52 B per particle (x0 16,
+    u 16, F write 16, row_ptr 4), plus the ADR damping terms and update per
+    particle (Integrate.cu:7-106): reads of F, F_old, v_half_old, u and b
+    (80 B), writes of v_half, v_half_old, u and F_old (64 B)."""
+    return 20.125 * nb + (52 + 144) * n
+
+
 def pmb_bytes(nb, n, dim=3):
     """Mixed PMB: col 4 + xi 4 + kappa 4 + fail 1/8 per bond; x0 8D + u 8D + f 4D + row_ptr 4."""
     return 12.125 * nb + (20 * dim + 4) * n
 
 
 def load_traffic(kernel_key):
+    """HBM bytes per launch of a kernel from profiles/traffic.json (rocprofv3
+    FETCH_SIZE and WRITE_SIZE runs, scaled by the copy calibration), with the
+    file's provenance: the commit and command lines it was measured with."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     try:
         with open(path) as fh:
             data = json.load(fh)
-        return data.get(kernel_key)
     except (OSError, ValueError):
-        return None
+        return None, None
+    kernels = data.get("kernels", {})
+    src = {"file": "profiles/traffic.json", "commit": data.get("commit"), "summary": data.get("summary")}
+    return kernels.get(kernel_key), src
 
 
 def cpu_model():
@@ -134,6 +149,57 @@ def cpu_baseline(sample_dims=(200, 200, 100), evals=40, one_thread_evals=2):
     }
 
 
+def cpu_full_configs(one_thread_steps=(50, 5, 1)):
+    """The FP64 oracle over the whole runs of the plate (C0, 1000 steps), the
+    Kalthoff-Winkler impact (C1, 1350 steps) and the LPS block (C2, 200
+    steps) on all host threads, and a few steps of each on one thread (the
+    reference's sequential CPU path).  About two minutes on a 16-thread host."""
+    from oracle import oracle as O
+    from oracle.runner import OracleModel
+    from pd_gpu1_amd import _lib, scenarios
+
+    rows = {}
+    cases = (("c0_plate_pmb", scenarios.c0_plate_pmb()),
+             ("c1_kalthoff_winkler", scenarios.c1_kalthoff_winkler()),
+             ("c2_lps_block", scenarios.c2_lps_block(precision=_lib.PD_PREC_FP64)))
+    for (name, sc), n1 in zip(cases, one_thread_steps):
+        t0 = time.perf_counter()
+        m = OracleModel(scenarios.params_dict(sc.p), sc.x, sc.y, sc.z, sc.n_int)
+        m.build_neighbors()
+        if sc.cyl is not None:
+            m.surface_correct()
+        m.init_state()
+        if sc.cyl is not None:
+            m.set_notches_kw(0.025)
+            m.cylinder_init(sc.cyl)
+        elif sc.p.model == _lib.PD_MODEL_LPS:
+            m.lps_weights()
+            m.V = [a.copy() for a in sc.v0]
+        else:
+            m.set_crack_2d(0.01, 0.0, 0.0, 0.0)
+        setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(sc.steps):
+            m.step()
+        run = time.perf_counter() - t0
+        threads = O.get_threads()
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        for _ in range(n1):
+            m.step()
+        run1 = time.perf_counter() - t0
+        O.set_threads(threads)
+        nb = len(m.col)
+        rows[name] = {"particles": len(sc.x), "bonds": nb, "steps": sc.steps, "setup_s": setup, "run_s": run,
+                      "bond_updates_per_s": nb * sc.steps / run, "threads": threads,
+                      "one_thread": {"steps": n1, "s": run1, "bond_updates_per_s": nb * n1 / run1},
+                      "broken_bonds_at_end": int((m.fail == 0).sum())}
+        print(f"[bench] oracle {name}: {sc.steps} steps in {run:.1f} s on {threads} threads", file=sys.stderr,
+              flush=True)
+        del m
+    return {"kind": "port", **rows, **cpu_model()}
+
+
 def batch_cpu_baseline(specs, du, tol, max_iters, T):
     """FP64 oracle ADR, one increment, on the given specimens one after the
     other (all OpenMP threads); the rate is bond-iterations per second."""
@@ -184,12 +250,15 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                     parts_used = b.parts_used()
                 active = its > 0
                 unconv = int(((res > 1e-6) & active).sum())
-                nbs = [b.spec_info(s_)[2] for s_ in range(len(specs))] if k == 0 else None
+                info = [b.spec_info(s_) for s_ in range(len(specs))] if k == 0 else None
                 incs.append({"wall_ms": round(ms, 1), "active": int(active.sum()),
                              "iterations_median": int(np.median(its[active])) if active.any() else 0,
                              "iterations_max": int(its.max()), "unconverged": unconv})
-                if nbs is not None:
-                    incs[-1]["bond_iterations"] = int(sum(nb * (int(i) + 1) for nb, i in zip(nbs, its)))
+                if info is not None:
+                    evals = [int(i) + 1 for i in its]  # force evaluations: iterations + the converged check
+                    incs[-1]["bond_iterations"] = int(sum(q[2] * e for q, e in zip(info, evals)))
+                    incs[-1]["algorithmic_bytes"] = float(sum(batch_iteration_bytes(q[2], q[0]) * e
+                                                              for q, e in zip(info, evals)))
                 if log:
                     print(f"[bench] batch {'warm' if mode else 'plain'} increment {k}: {ms:.0f} ms, "
                           f"{int(active.sum())} active, iterations median {incs[-1]['iterations_median']} "
@@ -197,7 +266,15 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
             runs["warm" if mode else "plain"] = incs
     tot = {k: sum(i["wall_ms"] for i in v) for k, v in runs.items()}
     # bond-iterations per second of increment 0 (plain), for the CPU comparison
-    gpu_rate = runs["plain"][0]["bond_iterations"] / (runs["plain"][0]["wall_ms"] / 1e3)
+    inc0 = runs["plain"][0]
+    gpu_rate = inc0["bond_iterations"] / (inc0["wall_ms"] / 1e3)
+    gbs = inc0["algorithmic_bytes"] / (inc0["wall_ms"] / 1e3) / 1e9
+    batch_roofline = {"bound": "hbm", "kernel": "k_batch_adr", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "bytes_per_specimen_iteration": "20.125 Nb + 196 N (FP64 PMB force 20.125 Nb + 52 N, "
+                                                      "ADR damping terms and update 144 N)",
+                      "workload": "increment 0 of the plain run (one launch), every specimen's force "
+                                  "evaluations = iterations + 1"}
     cpu = batch_cpu_baseline(specs[:16], W.DU, 1e-6, max_iters, block) if with_cpu else None
     out["batched"] = {"workload": f"{n_batch} PMB plates, sides 40..200 (seed 0x5EED0003+k), {batch_increments} "
                                   "increments of du=1e-6 m, ADR to relative residual 1e-6, one persistent launch "
@@ -208,7 +285,7 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                       "increments_plain": runs["plain"], "increments_warm": runs["warm"],
                       "launch_shape": {"block": block, "lanes_per_row": group,
                                        "workgroups_per_specimen": parts_used},
-                      "autotune_ms_per_iteration": tuned,
+                      "autotune_ms_per_iteration": tuned, "roofline": batch_roofline,
                       "increment0_bond_iterations_per_s": gpu_rate, "cpu_baseline": cpu}
     wpath = os.path.join(ROOT, "pd_gpu1_amd", "data", "mpnn_warmstart.npy")
     specs = W.eval_specs(n_eval)
@@ -279,14 +356,19 @@ def mpnn_section(n=300, reps=20):
                                     "frac_of_hbm": gathers / t_fwd / 1e9 / HBM_PEAK_GBS}}
 
 
-def pmb_section(steps=50, warmup=5, reps=10):
-    """PMB force on the same 4M-particle block geometry (mixed precision)."""
+def pmb_section(steps=50, warmup=5, reps=10, fixed=None):
+    """PMB force on the same 4M-particle block geometry (mixed precision);
+    fixed = (tile, block, lanes) skips the autotuner."""
     from pd_gpu1_amd import _lib, autotune, scenarios
 
     sc = scenarios.pmb_block(precision=_lib.PD_PREC_MIXED)
-    prob = sc.make(device=0)
-    shape = autotune.tune(prob)
-    prob.close()
+    if fixed:
+        t, b, g = fixed
+        shape = {"lds": True, "tile": t, "block": [b, b], "groups": [g, g], "ms": {}}
+    else:
+        prob = sc.make(device=0)
+        shape = autotune.tune(prob)
+        prob.close()
     prob = sc.make(device=0)
     autotune.apply(prob, shape)
     lds_active = prob.lds_info()[0]
@@ -302,13 +384,16 @@ def pmb_section(steps=50, warmup=5, reps=10):
     b = pmb_bytes(nb, N)
     gbs = b / (ms_force * 1e-3) / 1e9
     kname = ("k_pmb_lds<3,%d>" if lds_active else "k_pmb_mixed<3,%d>") % shape["groups"][0]
+    traffic, traffic_src = load_traffic(kname.split("<")[0])
     return {"workload": "3D PMB block 200x200x100 (4M particles, same jittered lattice), mixed precision, "
                         f"{steps} Verlet steps at dt=2e-8",
             "bonds": nb, "value": nb * steps / wall, "unit": "bond-updates/s", "ms_per_step": wall * 1e3 / steps,
             "force_bond_updates_per_s": nb / (ms_force * 1e-3), "groups": shape["groups"],
             "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": b,
-                         "ms_per_launch": ms_force, "traffic": load_traffic(kname.split("<")[0])}}
+                         "ms_per_launch": ms_force, "traffic": traffic, "traffic_source": traffic_src,
+                         "measured_frac": (round(traffic / (ms_force * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                           if traffic else None)}}
 
 
 def small_configs(cpu=True):
@@ -369,12 +454,17 @@ def main():
     ap.add_argument("--no-adr", action="store_true", help="skip the ADR / warm-start measurements")
     ap.add_argument("--no-pmb", action="store_true", help="skip the PMB block measurement")
     ap.add_argument("--no-small", action="store_true", help="skip the C0 plate and Kalthoff-Winkler runs")
+    ap.add_argument("--no-full-run", action="store_true", help="skip the whole 200-step run of the LPS block")
     ap.add_argument("--c3-full", action="store_true",
                     help="batched ADR at the full family size (512 plates x 50 increments, plain and warm: about 17 minutes)")
     ap.add_argument("--group", type=int, default=0, help="lanes per particle (0 = autotune)")
     ap.add_argument("--shape", default="", metavar="TILE,B0,B1,G0,G1",
                     help="fixed LDS launch shape instead of autotuning: tile edge (dx), workgroup size and "
                          "lanes per row of pass 0 and pass 1")
+    ap.add_argument("--pmb-shape", default="", metavar="TILE,BLOCK,G",
+                    help="fixed LDS launch shape of the PMB block instead of autotuning")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only time the FP64 oracle over the whole C0, C1 and C2 runs on the host (about 2 minutes)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter reference)")
     args = ap.parse_args()
@@ -382,6 +472,10 @@ def main():
         from pd_gpu1_amd.engine import stream_copy
         gbs = stream_copy(args.copy_calibration, reps=5)
         print(json.dumps({"copy_bytes": args.copy_calibration, "reps": 5, "launches": 6, "GB_per_s": gbs}))
+        return
+
+    if args.cpu_full:
+        print(json.dumps({"cpu_full_configs": cpu_full_configs()}))
         return
 
     if args.gpus != 1:
@@ -436,7 +530,25 @@ def main():
     broken = int((prob.fail_mask() == 0).sum())
     prob.close()
 
+    # The configuration's whole run from rest: 200 steps, the fracture that
+    # starts near step 100 included (breaks fall inside this timed region).
+    full_run = None
+    if not args.no_full_run:
+        prob = sc.make(device=0)
+        autotune.apply(prob, shape)
+        prob.sync()
+        t0 = time.perf_counter()
+        prob.run_verlet(sc.steps)
+        prob.sync()
+        wf = time.perf_counter() - t0
+        fb = int((prob.fail_mask() == 0).sum())
+        prob.close()
+        full_run = {"steps": sc.steps, "wall_s": wf, "ms_per_step": wf * 1e3 / sc.steps,
+                    "bond_updates_per_s": nb * sc.steps / wf, "broken_bonds_at_end": fb}
+        print(f"[bench] full {sc.steps}-step run in {wf:.3f} s, {fb} broken bonds", file=sys.stderr, flush=True)
+
     kname = "k_lps_force_lds<3,%d>" if lds_active else "k_lps_force_mixed<3,%d>"
+    traffic, traffic_src = load_traffic(kname.split("<")[0])
     roofline = {
         "bound": "hbm",
         "kernel": kname % groups[1],
@@ -444,7 +556,13 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(gbs_force / HBM_PEAK_GBS, 4),
-        "traffic": load_traffic(kname.split("<")[0]),
+        "traffic": traffic,
+        "traffic_source": traffic_src,
+        # what the profile shows limits the kernel: it moves fewer bytes than
+        # the algorithmic count (LDS-staged records, 2-byte index words), and
+        # its VALU and LDS pipes are the busy ones
+        "limiter": "valu+lds (rocprof counters; HBM traffic below the algorithmic bytes)",
+        "measured_frac": (round(traffic / (ms_force * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
         "bytes_per_launch": b_force,
         "ms_per_launch": ms_force,
         "kernel_stream_bytes_per_launch": kernel_stream_bytes(nb, N_int, lds=lds_active, recomputed=recomputed),
@@ -466,7 +584,8 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded jittered lattice)",
         "config": {"workload": "3D LPS block 200x200x100 (4M particles), dx=0.5mm, delta=3.015dx, "
-                               "mixed precision, 200 Verlet steps at dt=2e-8",
+                               f"mixed precision, {args.steps} timed Verlet steps at dt=2e-8 after {args.warmup} "
+                               f"untimed (the configuration runs {sc.steps}; its whole run is full_run)",
                    "particles": N, "bonds": nb, "parallelism": "single GPU",
                    "force_bond_updates_per_s": nb / ((ms_theta + ms_force) * 1e-3),
                    "groups": list(groups),
@@ -478,10 +597,12 @@ def main():
                    # still intact at the end gives a lower bound of the rate
                    "intact_bond_updates_per_s_lower_bound": (nb - broken) * args.steps / wall},
         "roofline": roofline,
+        "full_run": full_run,
     }
     if not args.no_pmb:
         t = time.perf_counter()
-        out["pmb_block"] = pmb_section()
+        out["pmb_block"] = pmb_section(fixed=tuple(int(v) for v in args.pmb_shape.split(",")) if args.pmb_shape
+                                       else None)
         print(f"[bench] PMB block {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_small:
         t = time.perf_counter()

@@ -10,7 +10,8 @@ workload), cfetch/ and cwrite/ (the same counters on a plain device copy of
 --copy-bytes bytes read and written, which calibrates the counters: bytes per
 launch = counter KiB * 1024 * copy bytes / copy counter bytes), and any other
 counter directories (their per-kernel means are listed).  traffic.json is
-written next to OUT.txt, keyed by kernel name fragment.
+written next to OUT.txt: the bytes per launch keyed by kernel name fragment,
+with the commit (--commit) and commands it was measured with.
 """
 import argparse
 import collections
@@ -40,7 +41,7 @@ def _mean_for(agg, fragment, counter=None):
     return sum(flat) / len(flat)
 
 
-def summarize(root, out_txt, kernels, copy_bytes, commands):
+def summarize(root, out_txt, kernels, copy_bytes, commands, commit=None):
     lines = ["rocprofv3 summary (generated by python -m pd_gpu1_amd.profsum)", ""]
     lines += ["Commands:"] + [f"  {c}" for c in commands] + [""]
     ks = _rows(os.path.join(root, "trace", "trace_kernel_stats.csv"))
@@ -84,7 +85,8 @@ def summarize(root, out_txt, kernels, copy_bytes, commands):
     with open(out_txt, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     with open(os.path.join(os.path.dirname(os.path.abspath(out_txt)), "traffic.json"), "w") as fh:
-        json.dump(traffic, fh, indent=1)
+        json.dump({"commit": commit, "summary": os.path.basename(out_txt), "commands": commands,
+                   "kernels": traffic}, fh, indent=1)
     return lines
 
 
@@ -95,8 +97,9 @@ def main(argv=None):
     ap.add_argument("kernels")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--commit", default=None, help="commit of the measured build (recorded in traffic.json)")
     a = ap.parse_args(argv)
-    print("\n".join(summarize(a.root, a.out, a.kernels.split(","), a.copy_bytes, a.command)))
+    print("\n".join(summarize(a.root, a.out, a.kernels.split(","), a.copy_bytes, a.command, a.commit)))
 
 
 if __name__ == "__main__":
