@@ -306,6 +306,7 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                                        "network": tot(first, 4), "network_fell_back": int(rows[first, 9].sum())},
         "later_increment_iterations": {"plain": tot(later, 2), "warm": tot(later, 3)},
         "network_weights": "pd_gpu1_amd/data/mpnn_warmstart.npy" if have_net else None,
+        "network_scale": W.NET_SCALE,
     }
     return out
 

@@ -284,10 +284,12 @@ int pd_batch_get_csr(pd_batch *b, int spec, int32_t *row_ptr, int32_t *col, uint
 /* ---- learned ADR warm start (no reference counterpart) --------------------
  * A message-passing network over the PD bond graph: 3 layers, hidden 64,
  * SiLU, BF16 MFMA operands with FP32 accumulation, trained by its own
- * forward/backward kernels with Adam.  Node features [N][8]: u (2), previous
- * increment (2), damage, strip flags (2), 1; edge features [E][3]: reference
- * bond vector / delta (2), intact flag.  Output [N][2]: the predicted
- * displacement increment (scaled as the caller scaled the targets). */
+ * forward/backward kernels with Adam.  Node features [N][16] (the caller's
+ * choice; pd_gpu1_amd/warmstart.py: position, previous increment, damage,
+ * strip flags, 1, and the node's coordinates in the frame of the pre-crack);
+ * edge features [E][3]: reference bond vector / delta (2), intact flag.
+ * Output [N][2]: the predicted displacement increment (scaled as the caller
+ * scaled the targets). */
 typedef struct pd_mpnn pd_mpnn;
 int pd_mpnn_create(int device, uint64_t seed, pd_mpnn **out);
 int pd_mpnn_destroy(pd_mpnn *m);

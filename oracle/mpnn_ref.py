@@ -2,7 +2,7 @@
 infrastructure only; the GPU network lives in pd_gpu1_amd/csrc/k_mpnn.hip).
 
 Network (no reference counterpart; the warm-start network of pd_gpu1_amd):
-    h0 = silu(x W0 + b0)                                   x: [N, 8]
+    h0 = silu(x W0 + b0)                                   x: [N, 16]
     for l in 1..3:
         P   = h W1_l                                       [N, 64]
         pre = P[col] + e Q_l + c_l                         per directed edge
@@ -16,7 +16,7 @@ W1, Q, c, W2, b2, then Wd, bd; matrices are [in][out] row-major.
 """
 import numpy as np
 
-F_IN, E_IN, HID, N_LAYERS, OUT = 8, 3, 64, 3, 2
+F_IN, E_IN, HID, N_LAYERS, OUT = 16, 3, 64, 3, 2
 
 
 def shapes():

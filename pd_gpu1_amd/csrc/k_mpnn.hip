@@ -268,7 +268,7 @@ __global__ void k_prep_weights(const float *__restrict__ th, MpnnWeights16 w)
     // W0^T padded: [64][32]
     if (t < 64 * 32) {
         const int n = t / 32, k = t % 32;
-        w.W0T[t] = (__bf16)(k < 8 ? th[L.W0 + k * 64 + n] : 0.f);
+        w.W0T[t] = (__bf16)(k < kNodeFeat ? th[L.W0 + k * 64 + n] : 0.f);
     }
     for (int l = 0; l < 3; ++l) {
         if (t < 64 * 64) {

@@ -105,7 +105,7 @@ void free_particles(pd_handle *h)
     dfree(h->row_ptr);
 }
 
-hipError_t alloc_set(D3 &s, int N)
+hipError_t alloc_set(D3 &s, int N, hipStream_t st)
 {
     double *p = nullptr;
     hipError_t e = hipMalloc((void **)&p, sizeof(double) * 3 * (size_t)(N > 0 ? N : 1));
@@ -113,7 +113,7 @@ hipError_t alloc_set(D3 &s, int N)
     s.p[0] = p;
     s.p[1] = p + N;
     s.p[2] = p + 2 * (size_t)N;
-    return hipMemsetAsync(p, 0, sizeof(double) * 3 * (size_t)N, 0);
+    return hipMemsetAsync(p, 0, sizeof(double) * 3 * (size_t)N, st);
 }
 
 // Staged neighbour lists of the LDS tile kernels for the current order
@@ -701,7 +701,7 @@ int pd_create(const pd_params *p, int device, pd_handle **out)
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
         hipMalloc((void **)&h->S, sizeof(DevScalars)) != hipSuccess ||
         hipMalloc((void **)&h->partials, sizeof(double) * 8 * h->red_blocks) != hipSuccess ||
-        hipMemset(h->S, 0, sizeof(DevScalars)) != hipSuccess) {
+        fill_on(h->stream, h->S, 0, sizeof(DevScalars)) != hipSuccess) {
         pd_destroy(h);
         return PD_E_HIP;
     }
@@ -768,25 +768,25 @@ int pd_set_positions(pd_handle *h, const double *x, const double *y, const doubl
     h->N = n;
     h->N_int = n_int;
     D3 *sets[] = {&h->x, &h->u, &h->v, &h->a, &h->b, &h->f, &h->vh, &h->vho, &h->fo};
-    for (D3 *s : sets) LAUNCH(alloc_set(*s, n));
+    for (D3 *s : sets) LAUNCH(alloc_set(*s, n, h->stream));
     {
         float *p = nullptr;
         LAUNCH(hipMalloc((void **)&p, sizeof(float) * 3 * (size_t)n));
-        LAUNCH(hipMemset(p, 0, sizeof(float) * 3 * (size_t)n));
+        LAUNCH(fill_on(h->stream, p, 0, sizeof(float) * 3 * (size_t)n));
         h->f32.p[0] = p;
         h->f32.p[1] = p + n;
         h->f32.p[2] = p + 2 * (size_t)n;
     }
     LAUNCH(dalloc(&h->dmg, n));
-    LAUNCH(hipMemset(h->dmg, 0, sizeof(double) * n));
+    LAUNCH(fill_on(h->stream, h->dmg, 0, sizeof(double) * n));
     LAUNCH(dalloc(&h->terms, 4 * (size_t)n));
     LAUNCH(dalloc(&h->m, n));
     LAUNCH(dalloc(&h->theta, n));
-    LAUNCH(hipMemset(h->theta, 0, sizeof(double) * n));
+    LAUNCH(fill_on(h->stream, h->theta, 0, sizeof(double) * n));
     LAUNCH(dalloc(&h->row_ptr, (size_t)n + 1));
     const double *src[3] = {x, y, z};
     for (int d = 0; d < h->P.dim; d++) {
-        LAUNCH(hipMemcpy(h->x.p[d], src[d], sizeof(double) * n, hipMemcpyHostToDevice));
+        LAUNCH(copy_on(h->stream, h->x.p[d], src[d], sizeof(double) * n, hipMemcpyHostToDevice));
         double lo = src[d][0], hi = src[d][0];
         for (int i = 1; i < n; i++) {
             lo = src[d][i] < lo ? src[d][i] : lo;
@@ -801,7 +801,7 @@ int pd_set_positions(pd_handle *h, const double *x, const double *y, const doubl
         h->lo[d] = 0.0;
         h->ncell[d] = 1;
     }
-    LAUNCH(hipDeviceSynchronize());
+    LAUNCH(hipStreamSynchronize(h->stream));
     return PD_OK;
 }
 
@@ -843,8 +843,8 @@ int pd_get_csr(pd_handle *h, int32_t *row_ptr, int32_t *col)
     if (!h) return PD_E_INVALID;
     REQUIRE(h->col, "pd_build_neighbors first");
     LAUNCH(hipStreamSynchronize(h->stream));
-    if (row_ptr) LAUNCH(hipMemcpy(row_ptr, h->row_ptr, sizeof(int32_t) * (h->N + 1), hipMemcpyDeviceToHost));
-    if (col) LAUNCH(hipMemcpy(col, h->col, sizeof(int32_t) * h->Nb, hipMemcpyDeviceToHost));
+    if (row_ptr) LAUNCH(copy_on(h->stream, row_ptr, h->row_ptr, sizeof(int32_t) * (h->N + 1), hipMemcpyDeviceToHost));
+    if (col) LAUNCH(copy_on(h->stream, col, h->col, sizeof(int32_t) * h->Nb, hipMemcpyDeviceToHost));
     return PD_OK;
 }
 
@@ -1339,13 +1339,13 @@ int pd_set_body_force_tensile(pd_handle *h, int nx, int ny)
     if (!h) return PD_E_INVALID;
     REQUIRE(nx > 0 && ny > 0 && (int64_t)nx * ny <= h->N, "nx*ny must fit in N");
     std::vector<double> bx((size_t)h->N, 0.0);
-    LAUNCH(hipMemcpy(bx.data(), h->b.p[0], sizeof(double) * h->N, hipMemcpyDeviceToHost));
+    LAUNCH(copy_on(h->stream, bx.data(), h->b.p[0], sizeof(double) * h->N, hipMemcpyDeviceToHost));
     const double E = h->P.emod, dx = h->P.dx;
     for (int i = 0; i < ny; i++) {
         bx[(size_t)i * nx] = -0.001 * E / dx;
         bx[(size_t)i * nx + nx - 1] = 0.001 * E / dx;
     }
-    LAUNCH(hipMemcpy(h->b.p[0], bx.data(), sizeof(double) * h->N, hipMemcpyHostToDevice));
+    LAUNCH(copy_on(h->stream, h->b.p[0], bx.data(), sizeof(double) * h->N, hipMemcpyHostToDevice));
     return PD_OK;
 }
 
@@ -1484,23 +1484,23 @@ int pd_get_field(pd_handle *h, int field, double *out)
     if (vec && field == PD_FIELD_F && use32(h)) {
         std::vector<float> tmp((size_t)D * h->N);
         for (int d = 0; d < D; d++)
-            LAUNCH(hipMemcpy(tmp.data() + (size_t)d * h->N, h->f32.p[d], sizeof(float) * h->N, hipMemcpyDeviceToHost));
+            LAUNCH(copy_on(h->stream, tmp.data() + (size_t)d * h->N, h->f32.p[d], sizeof(float) * h->N, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < tmp.size(); i++) out[i] = tmp[i];
         return PD_OK;
     }
     if (vec) {
         for (int d = 0; d < D; d++)
-            LAUNCH(hipMemcpy(out + (size_t)d * h->N, p + (size_t)d * h->N, sizeof(double) * h->N,
+            LAUNCH(copy_on(h->stream, out + (size_t)d * h->N, p + (size_t)d * h->N, sizeof(double) * h->N,
                              hipMemcpyDeviceToHost));
         return PD_OK;
     }
     if (field == PD_FIELD_THETA && use32(h) && h->theta32) {
         std::vector<float> tmp((size_t)h->N);
-        LAUNCH(hipMemcpy(tmp.data(), h->theta32, sizeof(float) * h->N, hipMemcpyDeviceToHost));
+        LAUNCH(copy_on(h->stream, tmp.data(), h->theta32, sizeof(float) * h->N, hipMemcpyDeviceToHost));
         for (int i = 0; i < h->N; i++) out[i] = tmp[i];
         return PD_OK;
     }
-    LAUNCH(hipMemcpy(out, p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    LAUNCH(copy_on(h->stream, out, p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return PD_OK;
 }
 
@@ -1517,17 +1517,17 @@ int pd_set_field(pd_handle *h, int field, const double *in)
     LAUNCH(hipStreamSynchronize(h->stream));
     if (vec) {
         for (int d = 0; d < h->P.dim; d++)
-            LAUNCH(hipMemcpy(p + (size_t)d * h->N, in + (size_t)d * h->N, sizeof(double) * h->N,
+            LAUNCH(copy_on(h->stream, p + (size_t)d * h->N, in + (size_t)d * h->N, sizeof(double) * h->N,
                              hipMemcpyHostToDevice));
         if (field == PD_FIELD_F && use32(h)) {
             std::vector<float> tmp((size_t)h->N);
             for (int d = 0; d < h->P.dim; d++) {
                 for (int i = 0; i < h->N; i++) tmp[i] = (float)in[(size_t)d * h->N + i];
-                LAUNCH(hipMemcpy(h->f32.p[d], tmp.data(), sizeof(float) * h->N, hipMemcpyHostToDevice));
+                LAUNCH(copy_on(h->stream, h->f32.p[d], tmp.data(), sizeof(float) * h->N, hipMemcpyHostToDevice));
             }
         }
     } else {
-        LAUNCH(hipMemcpy(p, in, sizeof(double) * n, hipMemcpyHostToDevice));
+        LAUNCH(copy_on(h->stream, p, in, sizeof(double) * n, hipMemcpyHostToDevice));
     }
     if (field == PD_FIELD_FAC || field == PD_FIELD_SCR || field == PD_FIELD_IDIST || field == PD_FIELD_W ||
         field == PD_FIELD_M)
@@ -1540,7 +1540,7 @@ int pd_get_fail_bits(pd_handle *h, uint32_t *out)
     if (!h || !out) return PD_E_INVALID;
     REQUIRE(h->fail, "pd_build_neighbors first");
     LAUNCH(hipStreamSynchronize(h->stream));
-    LAUNCH(hipMemcpy(out, h->fail, sizeof(uint32_t) * fail_words(h), hipMemcpyDeviceToHost));
+    LAUNCH(copy_on(h->stream, out, h->fail, sizeof(uint32_t) * fail_words(h), hipMemcpyDeviceToHost));
     return PD_OK;
 }
 
@@ -1550,7 +1550,7 @@ int pd_set_fail_bits(pd_handle *h, const uint32_t *in)
     REQUIRE(h->fail, "pd_build_neighbors first");
     LAUNCH(hipStreamSynchronize(h->stream));
     fail_changed(h);
-    LAUNCH(hipMemcpy(h->fail, in, sizeof(uint32_t) * fail_words(h), hipMemcpyHostToDevice));
+    LAUNCH(copy_on(h->stream, h->fail, in, sizeof(uint32_t) * fail_words(h), hipMemcpyHostToDevice));
     return PD_OK;
 }
 
@@ -1565,7 +1565,7 @@ int pd_cylinder_init(pd_handle *h, const pd_cyl *c)
     drop_graph(h);
     double st[12] = {0};
     st[1] = c->v;
-    LAUNCH(hipMemcpy(h->S->cyl, st, sizeof(st), hipMemcpyHostToDevice));
+    LAUNCH(copy_on(h->stream, h->S->cyl, st, sizeof(st), hipMemcpyHostToDevice));
     return PD_OK;
 }
 
@@ -1691,7 +1691,7 @@ int pd_checkpoint_read(pd_handle *h, const char *path, int64_t *step)
     if (ok) ok = std::fread(cyl, sizeof(cyl), 1, fp) == 1;
     std::fclose(fp);
     REQUIRE(ok, "checkpoint read failed or does not match this problem");
-    if (h->have_cyl) LAUNCH(hipMemcpy(h->S->cyl, cyl, sizeof(cyl), hipMemcpyHostToDevice));
+    if (h->have_cyl) LAUNCH(copy_on(h->stream, h->S->cyl, cyl, sizeof(cyl), hipMemcpyHostToDevice));
     if (step) *step = hdr[0];
     return PD_OK;
 }

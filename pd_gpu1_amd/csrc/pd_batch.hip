@@ -157,8 +157,8 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
     if (!rc) rc = alloc((void **)&b->d_iters, sizeof(int32_t) * nspec);
     if (!rc) rc = alloc((void **)&b->d_resid, sizeof(double) * nspec);
     if (!rc) rc = alloc((void **)&b->d_retired, sizeof(int32_t) * nspec);
-    if (!rc && hipMemset(b->d_retired, 0, sizeof(int32_t) * nspec) != hipSuccess) rc = PD_E_HIP;
-    if (!rc && hipMemset(b->buf, 0, sizeof(double) * 16 * b->n_total) != hipSuccess) rc = PD_E_HIP;
+    if (!rc && fill_on(b->stream, b->d_retired, 0, sizeof(int32_t) * nspec) != hipSuccess) rc = PD_E_HIP;
+    if (!rc && fill_on(b->stream, b->buf, 0, sizeof(double) * 16 * b->n_total) != hipSuccess) rc = PD_E_HIP;
     for (int s = 0; s < nspec && !rc; s++) {
         pd_handle *h = hs[s];
         const BatchSpec &d = b->spec[s];
@@ -172,33 +172,33 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
             rc = PD_E_HIP;
             break;
         }
-        (void)hipDeviceSynchronize();
+        (void)hipStreamSynchronize(h->stream);
         std::vector<uint32_t> fw((size_t)((d.nb + 31) / 32));
         if (!fw.empty()) pd_get_fail_bits(h, fw.data());
         hipError_t e = hipSuccess;
         auto cp = [&](void *dst, const void *src, size_t bytes) {
-            if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+            if (e == hipSuccess && bytes) e = copy_on(b->stream, dst, src, bytes, hipMemcpyDefault);
         };
         cp(b->buf + 0 * (size_t)b->n_total + d.p_off, h->x.p[0], sizeof(double) * d.n);
         cp(b->buf + 1 * (size_t)b->n_total + d.p_off, h->x.p[1], sizeof(double) * d.n);
         cp(b->row_ptr + d.rp_off, h->row_ptr, sizeof(int32_t) * (d.n + 1));
         cp(b->col + d.b_off, h->col, sizeof(int32_t) * d.nb);
-        if (e == hipSuccess) e = launch_pack_bond_pairs(nullptr, d.nb, h->kappa, h->idist, b->kd + d.b_off);
-        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = launch_pack_bond_pairs(b->stream, d.nb, h->kappa, h->idist, b->kd + d.b_off);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         cp(b->fail + d.b_off / 32, fw.data(), sizeof(uint32_t) * fw.size());
         if (e != hipSuccess) {
             b->err = std::string("batch copy: ") + hipGetErrorString(e);
             rc = PD_E_HIP;
         }
     }
-    if (!rc && hipMemcpy(b->d_spec, b->spec.data(), sizeof(BatchSpec) * nspec, hipMemcpyHostToDevice) != hipSuccess)
+    if (!rc && copy_on(b->stream, b->d_spec, b->spec.data(), sizeof(BatchSpec) * nspec, hipMemcpyHostToDevice) != hipSuccess)
         rc = PD_E_HIP;
     if (!rc) {
         // the largest plates set the wall time of an increment: hand them out first
         std::vector<int32_t> ord((size_t)nspec);
         for (int s = 0; s < nspec; ++s) ord[s] = s;
         std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) { return b->spec[a].nb > b->spec[c].nb; });
-        if (hipMemcpy(b->d_order, ord.data(), sizeof(int32_t) * nspec, hipMemcpyHostToDevice) != hipSuccess)
+        if (copy_on(b->stream, b->d_order, ord.data(), sizeof(int32_t) * nspec, hipMemcpyHostToDevice) != hipSuccess)
             rc = PD_E_HIP;
     }
     for (pd_handle *h : hs)
@@ -261,7 +261,7 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     if (W == 0) {
         std::vector<int32_t> ret((size_t)nspec, 0);
         if (b->retire)
-            B_TRY(hipMemcpy(ret.data(), b->d_retired, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
+            B_TRY(copy_on(b->stream, ret.data(), b->d_retired, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
         std::vector<int64_t> nb((size_t)nspec);
         std::vector<int32_t> np((size_t)nspec);
         for (int q = 0; q < nspec; ++q) {
@@ -335,7 +335,7 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     B_TRY(hipEventElapsedTime(&ms, e0, e1));
     if (W > 1) {
         int aborted = 0;
-        B_TRY(hipMemcpy(&aborted, b->g_abort, sizeof(int), hipMemcpyDeviceToHost));
+        B_TRY(copy_on(b->stream, &aborted, b->g_abort, sizeof(int), hipMemcpyDeviceToHost));
         if (aborted) {
             double *u0 = b->buf + (size_t)2 * b->n_total;
             B_TRY(hipMemcpyAsync(u0, b->snap, sizeof(double) * nfield, hipMemcpyDeviceToDevice, b->stream));
@@ -354,8 +354,8 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     b->last_block = block;
     b->last_parts = W;
     if (ms_out) *ms_out = ms;
-    if (iters_out) B_TRY(hipMemcpy(iters_out, b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
-    if (resid_out) B_TRY(hipMemcpy(resid_out, b->d_resid, sizeof(double) * nspec, hipMemcpyDeviceToHost));
+    if (iters_out) B_TRY(copy_on(b->stream, iters_out, b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
+    if (resid_out) B_TRY(copy_on(b->stream, resid_out, b->d_resid, sizeof(double) * nspec, hipMemcpyDeviceToHost));
     return PD_OK;
 }
 
@@ -469,7 +469,7 @@ int pd_batch_get_field(pd_batch *b, int spec, int field, double *out)
     const BatchSpec &d = b->spec[spec];
     B_TRY(hipStreamSynchronize(b->stream));
     for (int c = 0; c < 2; c++)
-        B_TRY(hipMemcpy(out + (size_t)c * d.n, b->buf + (size_t)(slot + c) * b->n_total + d.p_off,
+        B_TRY(copy_on(b->stream, out + (size_t)c * d.n, b->buf + (size_t)(slot + c) * b->n_total + d.p_off,
                         sizeof(double) * d.n, hipMemcpyDeviceToHost));
     return PD_OK;
 }
@@ -487,7 +487,7 @@ int pd_batch_set_field(pd_batch *b, int spec, int field, const double *in)
     const BatchSpec &d = b->spec[spec];
     B_TRY(hipStreamSynchronize(b->stream));
     for (int c = 0; c < 2; c++)
-        B_TRY(hipMemcpy(b->buf + (size_t)(slot + c) * b->n_total + d.p_off, in + (size_t)c * d.n,
+        B_TRY(copy_on(b->stream, b->buf + (size_t)(slot + c) * b->n_total + d.p_off, in + (size_t)c * d.n,
                         sizeof(double) * d.n, hipMemcpyHostToDevice));
     return PD_OK;
 }
@@ -497,10 +497,10 @@ int pd_batch_get_csr(pd_batch *b, int spec, int32_t *row_ptr, int32_t *col, uint
     if (!b || spec < 0 || spec >= (int)b->spec.size()) return PD_E_INVALID;
     const BatchSpec &d = b->spec[spec];
     B_TRY(hipStreamSynchronize(b->stream));
-    if (row_ptr) B_TRY(hipMemcpy(row_ptr, b->row_ptr + d.rp_off, sizeof(int32_t) * (d.n + 1), hipMemcpyDeviceToHost));
-    if (col) B_TRY(hipMemcpy(col, b->col + d.b_off, sizeof(int32_t) * d.nb, hipMemcpyDeviceToHost));
+    if (row_ptr) B_TRY(copy_on(b->stream, row_ptr, b->row_ptr + d.rp_off, sizeof(int32_t) * (d.n + 1), hipMemcpyDeviceToHost));
+    if (col) B_TRY(copy_on(b->stream, col, b->col + d.b_off, sizeof(int32_t) * d.nb, hipMemcpyDeviceToHost));
     if (fail_words)
-        B_TRY(hipMemcpy(fail_words, b->fail + d.b_off / 32, sizeof(uint32_t) * ((d.nb + 31) / 32),
+        B_TRY(copy_on(b->stream, fail_words, b->fail + d.b_off / 32, sizeof(uint32_t) * ((d.nb + 31) / 32),
                         hipMemcpyDeviceToHost));
     return PD_OK;
 }

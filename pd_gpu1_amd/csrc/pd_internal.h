@@ -13,6 +13,22 @@ struct CD3 { const double *p[3]; };
 struct F3 { float *p[3]; };
 struct CF3 { const float *p[3]; };
 
+// Stream-ordered copy and fill, waited for before they return.  Handles keep
+// all of their work on their own (non-blocking) stream and never use the
+// legacy stream: HIP refuses a legacy-stream operation while any stream is
+// capturing a graph, so this is what lets calls on different handles run
+// from different host threads while one of them captures its step graph.
+static inline hipError_t copy_on(hipStream_t s, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+static inline hipError_t fill_on(hipStream_t s, void *dst, int value, size_t bytes)
+{
+    hipError_t e = hipMemsetAsync(dst, value, bytes, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 static inline CD3 cd3(const D3 &a) { return CD3{{a.p[0], a.p[1], a.p[2]}}; }
 static inline CF3 cf3(const F3 &a) { return CF3{{a.p[0], a.p[1], a.p[2]}}; }
 

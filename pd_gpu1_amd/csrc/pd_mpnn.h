@@ -2,7 +2,9 @@
 #pragma once
 #include "pd_internal.h"
 
-// Offsets of the parameter blocks in the flat FP32 vector: W0 [8][64], b0,
+constexpr int kNodeFeat = 16;  // node features (the encoder GEMM pads them to K = 32)
+
+// Offsets of the parameter blocks in the flat FP32 vector: W0 [16][64], b0,
 // then per layer W1 [64][64], Q [3][64], c [64], W2 [128][64], b2 [64], then
 // Wd [64][2], bd [2] (matrices [in][out] row-major).
 struct ParamLayout {
@@ -11,7 +13,7 @@ struct ParamLayout {
     {
         ParamLayout L{};
         int o = 0;
-        L.W0 = o; o += 8 * 64;
+        L.W0 = o; o += kNodeFeat * 64;
         L.b0 = o; o += 64;
         for (int l = 0; l < 3; ++l) {
             L.W1[l] = o; o += 64 * 64;

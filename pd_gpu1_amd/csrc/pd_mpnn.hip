@@ -186,7 +186,7 @@ int backward(pd_mpnn *m)
     }
     // encoder: du0 = dh * silu'(u0)
     M_TRY(mpnn_dsilu_mul(s, n64, m->dh, m->u0, m->dz, m->dz16));
-    M_TRY(mpnn_gemm_tn(s, 32, 64, N, m->x16, m->dz16, m->part, chunks, kChunk, g + L.W0, 8, 64, 64));
+    M_TRY(mpnn_gemm_tn(s, 32, 64, N, m->x16, m->dz16, m->part, chunks, kChunk, g + L.W0, kNodeFeat, 64, 64));
     M_TRY(mpnn_colsum(s, N, 64, m->dz, m->part, cchunks, 1024, 64, g + L.b0));
     return PD_OK;
 }
@@ -249,15 +249,15 @@ int pd_mpnn_create(int device, uint64_t seed, pd_mpnn **out)
         }
     };
     const ParamLayout &L = m->L;
-    fill(L.W0, 8, 8 * 64);
+    fill(L.W0, kNodeFeat, kNodeFeat * 64);
     for (int l = 0; l < 3; ++l) {
         fill(L.W1[l], 64, 64 * 64);
         fill(L.Q[l], 3, 3 * 64);
         fill(L.W2[l], 128, 128 * 64);
     }
     fill(L.Wd, 64, 64 * 2);
-    if (hipMemcpy(m->theta, th.data(), sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(m->am, 0, sizeof(float) * n) != hipSuccess || hipMemset(m->av, 0, sizeof(float) * n) != hipSuccess)
+    if (copy_on(m->s, m->theta, th.data(), sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        fill_on(m->s, m->am, 0, sizeof(float) * n) != hipSuccess || fill_on(m->s, m->av, 0, sizeof(float) * n) != hipSuccess)
         return fail();
     *out = m;
     return PD_OK;
@@ -288,7 +288,7 @@ int pd_mpnn_get_params(pd_mpnn *m, float *out)
 {
     if (!m || !out) return PD_E_INVALID;
     M_TRY(hipStreamSynchronize(m->s));
-    M_TRY(hipMemcpy(out, m->theta, sizeof(float) * m->L.total, hipMemcpyDeviceToHost));
+    M_TRY(copy_on(m->s, out, m->theta, sizeof(float) * m->L.total, hipMemcpyDeviceToHost));
     return PD_OK;
 }
 
@@ -296,9 +296,9 @@ int pd_mpnn_set_params(pd_mpnn *m, const float *in)
 {
     if (!m || !in) return PD_E_INVALID;
     M_TRY(hipStreamSynchronize(m->s));
-    M_TRY(hipMemcpy(m->theta, in, sizeof(float) * m->L.total, hipMemcpyHostToDevice));
-    M_TRY(hipMemset(m->am, 0, sizeof(float) * m->L.total));
-    M_TRY(hipMemset(m->av, 0, sizeof(float) * m->L.total));
+    M_TRY(copy_on(m->s, m->theta, in, sizeof(float) * m->L.total, hipMemcpyHostToDevice));
+    M_TRY(fill_on(m->s, m->am, 0, sizeof(float) * m->L.total));
+    M_TRY(fill_on(m->s, m->av, 0, sizeof(float) * m->L.total));
     m->adam_t = 0;
     return PD_OK;
 }
@@ -308,7 +308,6 @@ int pd_mpnn_set_graph(pd_mpnn *m, int n_nodes, const int32_t *row_ptr, int64_t n
     // Symmetric CSR graph (j in N(i) <=> i in N(j)), rows ascending -- the PD
     // bond list.  The reverse-edge index turns the backward scatter into a gather.
     if (!m || n_nodes <= 0 || !row_ptr || (n_edges > 0 && !col)) return PD_E_INVALID;
-    M_TRY(hipStreamSynchronize(m->s));  // the uploads below go through the null stream
     if (row_ptr[0] != 0 || row_ptr[n_nodes] != n_edges) {
         m->err = "row_ptr does not match n_edges";
         return PD_E_INVALID;
@@ -333,41 +332,43 @@ int pd_mpnn_set_graph(pd_mpnn *m, int n_nodes, const int32_t *row_ptr, int64_t n
             rev[k] = (int32_t)(hit - col);
         }
     }
+    M_TRY(hipStreamSynchronize(m->s));  // no kernel may still read what ensure_capacity replaces
     int rc = ensure_capacity(m, n_nodes, n_edges);
     if (rc) return rc;
     m->N = n_nodes;
     m->E = n_edges;
-    M_TRY(hipMemcpy(m->rp, row_ptr, sizeof(int32_t) * (n_nodes + 1), hipMemcpyHostToDevice));
+    M_TRY(copy_on(m->s, m->rp, row_ptr, sizeof(int32_t) * (n_nodes + 1), hipMemcpyHostToDevice));
     if (n_edges) {
-        M_TRY(hipMemcpy(m->col, col, sizeof(int32_t) * n_edges, hipMemcpyHostToDevice));
-        M_TRY(hipMemcpy(m->rev, rev.data(), sizeof(int32_t) * n_edges, hipMemcpyHostToDevice));
+        M_TRY(copy_on(m->s, m->col, col, sizeof(int32_t) * n_edges, hipMemcpyHostToDevice));
+        M_TRY(copy_on(m->s, m->rev, rev.data(), sizeof(int32_t) * n_edges, hipMemcpyHostToDevice));
     }
-    M_TRY(hipMemcpy(m->dinv, dinv.data(), sizeof(float) * n_nodes, hipMemcpyHostToDevice));
+    M_TRY(copy_on(m->s, m->dinv, dinv.data(), sizeof(float) * n_nodes, hipMemcpyHostToDevice));
     return PD_OK;
 }
 
 int pd_mpnn_set_features(pd_mpnn *m, const float *node_feat, const float *edge_feat, const float *target,
                          const float *weight)
 {
-    // node_feat [N][8], edge_feat [E][3]; target [N][2] and weight [N] may be
-    // NULL for inference.
+    // node_feat [N][kNodeFeat], edge_feat [E][3]; target [N][2] and weight [N]
+    // may be NULL for inference.
     if (!m || !node_feat || (m->E > 0 && !edge_feat) || m->N <= 0) return PD_E_INVALID;
     M_TRY(hipStreamSynchronize(m->s));
     const int N = m->N;
     std::vector<__bf16> x((size_t)N * 32);
     for (int i = 0; i < N; ++i)
-        for (int c = 0; c < 32; ++c) x[(size_t)i * 32 + c] = (__bf16)(c < 8 ? node_feat[(size_t)i * 8 + c] : 0.f);
-    M_TRY(hipMemcpy(m->x16, x.data(), sizeof(__bf16) * x.size(), hipMemcpyHostToDevice));
-    if (m->E) M_TRY(hipMemcpy(m->ef, edge_feat, sizeof(float) * 3 * m->E, hipMemcpyHostToDevice));
+        for (int c = 0; c < 32; ++c)
+            x[(size_t)i * 32 + c] = (__bf16)(c < kNodeFeat ? node_feat[(size_t)i * kNodeFeat + c] : 0.f);
+    M_TRY(copy_on(m->s, m->x16, x.data(), sizeof(__bf16) * x.size(), hipMemcpyHostToDevice));
+    if (m->E) M_TRY(copy_on(m->s, m->ef, edge_feat, sizeof(float) * 3 * m->E, hipMemcpyHostToDevice));
     if (target && weight) {
-        M_TRY(hipMemcpy(m->tgt, target, sizeof(float) * 2 * (size_t)N, hipMemcpyHostToDevice));
-        M_TRY(hipMemcpy(m->wt, weight, sizeof(float) * N, hipMemcpyHostToDevice));
+        M_TRY(copy_on(m->s, m->tgt, target, sizeof(float) * 2 * (size_t)N, hipMemcpyHostToDevice));
+        M_TRY(copy_on(m->s, m->wt, weight, sizeof(float) * N, hipMemcpyHostToDevice));
         double ws = 0.0;
         for (int i = 0; i < N; ++i) ws += weight[i];
         m->wsum = ws;
     } else {
         std::vector<float> ones(N, 1.f);
-        M_TRY(hipMemcpy(m->wt, ones.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+        M_TRY(copy_on(m->s, m->wt, ones.data(), sizeof(float) * N, hipMemcpyHostToDevice));
         m->wsum = N;
     }
     return PD_OK;

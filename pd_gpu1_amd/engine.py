@@ -227,10 +227,14 @@ class Problem:
         self._c(lib().pd_adr_iterate(self._h, ct, None, None))
         return None
 
-    def adr_solve(self, ct0, max_iters, tol):
+    def adr_solve(self, ct0, max_iters, tol, nonfinite_ok=False):
+        """ADR iterations until the relative residual is <= tol or max_iters
+        ran: (iterations, residual, converged).  A non-finite state raises
+        PdError unless nonfinite_ok, which reports it as not converged."""
         it, r = C.c_int(), C.c_double()
         rc = lib().pd_adr_solve(self._h, ct0, max_iters, tol, C.byref(it), C.byref(r))
-        if rc not in (_lib.PD_OK, _lib.PD_E_NOT_CONVERGED):
+        ok_codes = (_lib.PD_OK, _lib.PD_E_NOT_CONVERGED) + ((_lib.PD_E_NONFINITE,) if nonfinite_ok else ())
+        if rc not in ok_codes:
             self._c(rc)
         return it.value, r.value, rc == _lib.PD_OK
 

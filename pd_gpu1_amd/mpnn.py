@@ -7,7 +7,7 @@ import numpy as np
 from . import _lib
 from ._lib import lib
 
-N_FEAT, E_FEAT, OUT = 8, 3, 2
+N_FEAT, E_FEAT, OUT = 16, 3, 2
 
 
 def _f32(a):

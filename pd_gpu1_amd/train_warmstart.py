@@ -33,23 +33,34 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0x5EED0004)
     ap.add_argument("--out", default="pd_gpu1_amd/data/mpnn_warmstart.npy")
     ap.add_argument("--log", default=None)
+    ap.add_argument("--chunk", type=int, default=512, help="plates per batched-ADR launch")
+    ap.add_argument("--init", default=None, help="start from these parameters (.npy) instead of the seeded ones")
     a = ap.parse_args(argv)
 
     def log(msg):
         print(msg, flush=True)
 
     specs = PB.plate_family(a.plates, seed_base=a.seed, n_min=a.n_min, n_max=a.n_max, dx=W.DX)
-    recs, iters, t_data = W.generate(specs, a.increments, log=log)
+    recs, iters, t_data = [], [], 0.0
+    for k in range(0, a.plates, a.chunk):  # one batched launch per chunk of plates
+        r, it, td = W.generate(specs[k:k + a.chunk], a.increments, log=log)
+        recs += r
+        iters.append(it)
+        t_data += td
+    iters = np.concatenate(iters, axis=1)
     log(f"[data] {a.plates} plates x {a.increments} increments in {t_data:.1f} s")
     p = PB.plate_params(W.DX)
     t0 = time.perf_counter()
     with Mpnn(seed=a.seed) as net:
+        if a.init:
+            net.set_params(np.load(a.init))
         hist = W.train(net, recs, p.delta, a.epochs, lr=a.lr, lr_end=a.lr_end, log=log)
         theta = net.get_params()
     t_train = time.perf_counter() - t0
     np.save(a.out, theta.astype(np.float32))
-    info = dict(plates=a.plates, increments=a.increments, samples=a.plates * a.increments, epochs=a.epochs,
-                n_range=[a.n_min, a.n_max], seed=a.seed, lr=[a.lr, a.lr_end], loss_per_epoch=hist,
+    info = dict(command="python -m pd_gpu1_amd.train_warmstart " + " ".join(argv if argv is not None else sys.argv[1:]),
+                plates=a.plates, increments=a.increments, samples=a.plates * a.increments, epochs=a.epochs,
+                n_range=[a.n_min, a.n_max], seed=a.seed, lr=[a.lr, a.lr_end], init=a.init, loss_per_epoch=hist,
                 data_seconds=t_data, train_seconds=t_train,
                 adr_iterations_median_per_increment=[int(np.median(r)) for r in iters])
     with open(a.log or a.out.replace(".npy", ".json"), "w") as f:

@@ -13,22 +13,23 @@ start also moves the free particles by a predicted increment:
 * for the first increment of a plate (no history) and after an increment in
   which bonds broke, the prediction is the uniform-stretch field between the
   strips, du_base = (0, du * y / y_strip), plus the correction the
-  message-passing network (pd_mpnn_*) predicts from the plate's bond graph:
-  du_pred = du_base + du * (0, y_net_y).  Without the network (the baseline
-  the network is compared with) du_base alone is used.  Only the y part of
-  the prediction is applied: measured on unseen plates, any error in the
-  x displacement of the guess (even with rigid-body modes removed) leaves a
-  slowly damped mode that multiplies the ADR iteration count by 4-20, while
-  the y part alone is harmless.
+  message-passing network (pd_mpnn_*) predicts from the plate's bond graph.
+  Without the network (the baseline the network is compared with) du_base
+  alone is used.  The loaded plate with its centred crack is symmetric under
+  the point reflection p -> -p about the plate centre, so while the broken
+  bonds are too its solution is odd, u(-p) = -u(p); the network output is
+  projected onto that odd part before it is applied.  An even part in the
+  guess (a rigid-body rotation, for one) is pure error that ADR damps only
+  slowly; removing it is what lets the x correction help instead of hurt.
 
 ADR then converges from that guess to the same tolerance, so accuracy is
-unchanged and only the iteration count moves.  Fallback: when the warm run's
-residual after ``probe`` iterations exceeds the residual plain ADR had after
-the same number of iterations, the increment restarts from the plain initial
-state.
+unchanged and only the iteration count moves.  Fallback: a second handle of
+the same plate runs plain ADR from the plain initial state at the same time,
+and the first of the two runs to reach tol gives the increment.
 
-Node features [N][8]: position relative to the plate centre / half side (2),
-du_prev / du (2), damage, top-strip flag, bottom-strip flag, 1.  Edge
+Node features [N][16]: position relative to the plate centre / half side (2),
+du_prev / du (2), damage, top-strip flag, bottom-strip flag, 1, and eight
+features of the node against the pre-crack (crack_features).  Edge
 features [E][3]: reference bond vector / delta (2), intact flag.  Targets
 [N][2]: (du_next - du_base) / du, weighted 1 on free particles and 0 on the
 strips (their motion is prescribed).  Training samples are the increments the network is
@@ -46,6 +47,13 @@ from .mpnn import Mpnn
 
 DU = 1.0e-6
 DX = 0.5e-3
+# Factor on the network's correction.  Chosen on validation plates (index
+# 128-191 of the evaluation family, disjoint from the plates the evaluation
+# reports): the full correction gives the lowest median iteration count but a
+# heavy tail (3.3x the uniform-stretch start's iterations on one plate in 64,
+# worse on 16); 0.75 was worse on 4 of 64, by 1.13x at most, and cut the
+# total by 21%.
+NET_SCALE = 0.75
 
 
 def plate_layout(n):
@@ -70,10 +78,45 @@ def base_increment(x, y, n_int, top):
     return b
 
 
+def plate_half(x, y, n_int):
+    """Half side of the free plate (the unit of the normalized positions)."""
+    return 0.5 * max(np.ptp(x[:n_int]), np.ptp(y[:n_int]), DX)
+
+
 def normalized_positions(x, y, n_int):
     cx, cy = x[:n_int].mean(), y[:n_int].mean()
-    half = 0.5 * max(np.ptp(x[:n_int]), np.ptp(y[:n_int]), DX)
+    half = plate_half(x, y, n_int)
     return np.stack([(x - cx) / half, (y - cy) / half])
+
+
+def crack_features(pos, clen, angle, half):
+    """[8][N] description of each node against the centred pre-crack (the
+    global context a few message-passing layers cannot see: the crack's
+    perturbation of the displacement field spans its whole length).  With
+    (s, t) the node's coordinates along and across the crack in units of the
+    crack half-length a: s/3 and t/3 clipped to [-1, 1]; the opening profile
+    of a straight crack under remote tension, sign(t) sqrt(1 - s^2) for
+    |s| < 1 (0 beyond the tips); a / half side; cos and sin of the crack
+    angle; 1 / (1 + r^2) with r the distance to the nearer tip; and
+    1 / (1 + s^2 + t^2).  All zero for a plate without a crack."""
+    N = pos.shape[1]
+    out = np.zeros((8, N))
+    a = 0.5 * clen / half
+    if not clen > 0.0:
+        return out
+    c, sn = math.cos(angle), math.sin(angle)
+    s = (pos[0] * c + pos[1] * sn) / a
+    t = (-pos[0] * sn + pos[1] * c) / a
+    out[0] = np.clip(s / 3.0, -1.0, 1.0)
+    out[1] = np.clip(t / 3.0, -1.0, 1.0)
+    out[2] = np.sign(t) * np.sqrt(np.clip(1.0 - s * s, 0.0, None))
+    out[3] = a
+    out[4] = c
+    out[5] = sn
+    r2 = np.minimum((s - 1.0) ** 2, (s + 1.0) ** 2) + t * t
+    out[6] = 1.0 / (1.0 + r2)
+    out[7] = 1.0 / (1.0 + s * s + t * t)
+    return out
 
 
 def edge_features(x, y, row_ptr, col, mask, delta):
@@ -93,16 +136,18 @@ def damage(row_ptr, mask):
     return np.where(deg > 0, 1.0 - intact / np.maximum(deg, 1), 0.0)
 
 
-def node_features(pos, du_prev, dmg, top, bot):
-    """pos: [2][N] coordinates relative to the plate centre / half side."""
+def node_features(pos, du_prev, dmg, top, bot, crack):
+    """pos: [2][N] coordinates relative to the plate centre / half side;
+    crack: [8][N] from crack_features."""
     N = pos.shape[1]
-    f = np.empty((N, 8), np.float32)
+    f = np.empty((N, 16), np.float32)
     f[:, 0:2] = pos.T
     f[:, 2:4] = (du_prev / DU).T
     f[:, 4] = dmg
     f[:, 5] = top
     f[:, 6] = bot
     f[:, 7] = 1.0
+    f[:, 8:16] = crack.T
     return f
 
 
@@ -116,10 +161,10 @@ def generate(specs, n_inc, tol=1e-6, max_iters=400000, block=1024, group=4, log=
     t0 = time.perf_counter()
     with PB.Batch(p, specs) as b:
         b.set_group(group)
-        for s, (n, _, _, _) in enumerate(specs):
+        for s, (n, _, clen, ang) in enumerate(specs):
             rp, col, mask = b.csr(s)
             x, y, n_int, top, bot = plate_layout(n)
-            recs.append(dict(n=n, n_int=n_int, rp=rp, col=col, x=x, y=y, top=top, bot=bot,
+            recs.append(dict(n=n, n_int=n_int, rp=rp, col=col, x=x, y=y, top=top, bot=bot, clen=clen, angle=ang,
                              u=[np.zeros((2, len(x)))], mask=[mask]))
         iters = []
         for m in range(n_inc):
@@ -147,13 +192,14 @@ def samples(rec, delta):
     free = np.zeros(len(rec["x"]), np.float32)
     free[: rec["n_int"]] = 1.0
     pos = normalized_positions(rec["x"], rec["y"], rec["n_int"])
+    cf = crack_features(pos, rec["clen"], rec["angle"], plate_half(rec["x"], rec["y"], rec["n_int"]))
     base = base_increment(rec["x"], rec["y"], rec["n_int"], rec["top"])
     for k in range(1, len(us)):
         if not uses_network(k - 1, masks[k - 1], masks[k - 2] if k >= 2 else None):
             continue
         prev = us[k - 1] - us[k - 2] if k >= 2 else np.zeros_like(us[0])
         mask = masks[k - 1]
-        x = node_features(pos, prev, damage(rec["rp"], mask), rec["top"], rec["bot"])
+        x = node_features(pos, prev, damage(rec["rp"], mask), rec["top"], rec["bot"], cf)
         ef = edge_features(rec["x"], rec["y"], rec["rp"], rec["col"], mask, delta)
         t = ((us[k] - us[k - 1] - base) / DU).T.astype(np.float32)
         yield x, ef, t, free
@@ -185,7 +231,12 @@ def train(net, recs, delta, epochs, lr=1e-3, lr_end=1e-4, seed=0, log=None):
 
 # --------------------------------------------------------------- evaluation
 class Plate:
-    """One pre-cracked plate on its own pd_handle, loaded increment by increment."""
+    """One pre-cracked plate on its own pd_handle, loaded increment by increment.
+
+    Set-up also prepares, once per plate, what the network path reuses every
+    time it predicts: the static node and edge features (positions, the
+    pre-crack frame, strip flags, reference bond vectors) and the point
+    reflection maps (mirror_maps)."""
 
     def __init__(self, n, crack_len, angle, device=0):
         p = PB.plate_params(DX)
@@ -200,13 +251,27 @@ class Plate:
         self.delta = p.delta
         self.rp, self.col = self.prob.csr()
         self.pos = normalized_positions(self.x, self.y, self.n_int)
+        self.cf = crack_features(self.pos, crack_len, angle, plate_half(self.x, self.y, self.n_int))
         self.base = base_increment(self.x, self.y, self.n_int, self.top)
         self.du_prev = np.zeros((2, len(self.x)))
         self.mask_prev = None  # intact mask before the previous increment
         self.k = 0
+        self._static_x = node_features(self.pos, np.zeros((2, len(self.x))), np.zeros(len(self.x)), self.top,
+                                       self.bot, self.cf)
+        self._ef = edge_features(self.x, self.y, self.rp, self.col, np.ones(len(self.col), np.uint8), self.delta)
+        self._mir = None
+        self.mirror_maps()
 
     def close(self):
         self.prob.close()
+
+    def features(self, mask):
+        """Node and edge features of the current state (intact mask `mask`)."""
+        x = self._static_x.copy()
+        x[:, 2:4] = (self.du_prev / DU).T
+        x[:, 4] = damage(self.rp, mask)
+        self._ef[:, 2] = mask
+        return x, self._ef
 
     def _solve(self, tol, max_iters, probe):
         it1, r1, ok = self.prob.adr_solve(0, min(probe, max_iters), tol)
@@ -227,45 +292,88 @@ class Plate:
         self.k += 1
         return its, r, r_probe, sec
 
-    def step_warm(self, net, r_plain_probe, tol=1e-6, max_iters=400000, probe=50, use_net=True):
-        """Warm-started increment: du_prev, or where uses_network says so the
-        uniform-stretch increment plus (when use_net) the network's
-        correction.  Returns (iterations, residual, seconds, fell_back,
-        predict seconds, used_net)."""
+    def mirror_maps(self):
+        """Point reflection p -> -p about the plate centre, which maps the
+        lattice, the centred crack and the strip loading (top +du, bottom
+        -du) onto themselves with the displacement reversed: the particle
+        index of each particle's mirror image and the CSR position of each
+        bond's mirror bond (computed once per plate)."""
+        if self._mir is None:
+            cx, cy = self.x[: self.n_int].mean(), self.y[: self.n_int].mean()
+            ix = np.rint((self.x - cx) / DX * 2.0).astype(np.int64)
+            iy = np.rint((self.y - cy) / DX * 2.0).astype(np.int64)
+            M = 4 * (int(max(np.abs(ix).max(), np.abs(iy).max())) + 2)
+            key = (ix + M) * (2 * M + 1) + (iy + M)
+            order = np.argsort(key)
+            mkey = (-ix + M) * (2 * M + 1) + (-iy + M)
+            pos = np.searchsorted(key[order], mkey)
+            mir = order[np.minimum(pos, len(order) - 1)]
+            assert np.array_equal(key[mir], mkey), "the layout is not point-symmetric"
+            N = len(self.x)
+            rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(self.rp))
+            bkey = rows * N + self.col
+            bmir = np.searchsorted(bkey, mir[rows] * N + mir[self.col])
+            assert np.array_equal(bkey[bmir], mir[rows] * N + mir[self.col])
+            self._mir, self._bmir = mir, bmir
+        return self._mir, self._bmir
+
+    def step_warm(self, net, tol=1e-6, max_iters=400000, use_net=True, fallback=None, chunk=256):
+        """Warm-started increment: the previous increment du_prev, or where
+        uses_network says so the uniform-stretch increment plus (when use_net)
+        the network's correction.  The problem is symmetric under the point
+        reflection about the plate centre while the broken bonds are, so its
+        solution is odd under it; the network's correction is projected onto
+        that odd part (its even part, which is pure error, is removed) and the
+        network is used only while the symmetry holds.
+
+        Fallback to plain ADR: when the run from the guess does not reach tol
+        within max_iters (or its state goes non-finite), the increment
+        restarts from the plain start (previous solution, strips moved) and
+        the iterations of both runs are counted.  With `fallback` (a Plate of
+        the same specimen, used only as a second solver) plain ADR instead
+        runs from the plain start at the same time, on its own handle and
+        stream, while this plate runs from the guess, both in chunks of
+        `chunk` iterations on two host threads; the first to reach tol gives
+        the increment (the other stops at its next chunk boundary) and the
+        result is this plate's state.  Returns (iterations, residual,
+        seconds, fell_back, predict seconds, used_net)."""
         t = time.perf_counter()
         u0 = self.prob.get("U")
         mask0 = self.prob.fail_mask()
-        guess = u0 + self.du_prev
-        fresh = uses_network(self.k, mask0, self.mask_prev)
-        used_net = use_net and fresh
-        if fresh:
-            guess = u0 + self.base
-        if used_net:
-            x = node_features(self.pos, self.du_prev, damage(self.rp, mask0), self.top, self.bot)
-            net.set_features(x, edge_features(self.x, self.y, self.rp, self.col, mask0, self.delta))
-            guess[1, : self.n_int] += DU * net.forward()[: self.n_int, 1].astype(np.float64)
-        t_pred = time.perf_counter() - t
         ni, nt = self.n_int, 3 * (len(self.x) - self.n_int) // 6
-        guess[:, ni:] = u0[:, ni:]
-        guess[1, ni:ni + nt] += DU
-        guess[1, ni + nt:] -= DU
-        self.prob.set("U", guess)
-        its, r, _ = self.prob.adr_solve(0, probe, tol)
+        start = u0.copy()
+        start[1, ni:ni + nt] += DU
+        start[1, ni + nt:] -= DU
+        fresh = uses_network(self.k, mask0, self.mask_prev)
+        guess = u0 + (self.base if fresh else self.du_prev)
+        used_net = False
+        if use_net and net is not None and fresh:
+            mir, bmir = self.mirror_maps()
+            if np.array_equal(mask0, mask0[bmir]):
+                used_net = True
+                x, ef = self.features(mask0)
+                net.set_features(x, ef)
+                y_net = net.forward().astype(np.float64).T
+                y_odd = 0.5 * (y_net - y_net[:, mir])
+                guess[:, :ni] += (NET_SCALE * DU) * y_odd[:, :ni]
+        t_pred = time.perf_counter() - t
+        guess[:, ni:] = start[:, ni:]
         fell_back = False
-        if r > tol and r > r_plain_probe:
-            # the guess is doing worse than plain ADR did after as many
-            # iterations: restart this increment from the plain initial state
-            fell_back = True
-            self.prob.set_fail_mask(mask0)
-            start = u0.copy()
-            start[1, ni:ni + nt] += DU
-            start[1, ni + nt:] -= DU
-            self.prob.set("U", start)
-            it2, r, _ = self._solve(tol, max_iters, probe)
-            its += it2
-        elif r > tol:
-            it2, r, _ = self.prob.adr_solve(probe, max_iters - probe, tol)
-            its += it2
+        if used_net and fallback is not None:
+            fallback.prob.set_fail_mask(mask0)
+            its, r, fell_back = race(self, guess, fallback, start, tol, max_iters, chunk)
+            if fell_back:
+                self.prob.set("U", fallback.prob.get("U"))
+                self.prob.set_fail_mask(fallback.prob.fail_mask())
+        else:
+            self.prob.set("U", guess)
+            its, r, ok = self.prob.adr_solve(0, max_iters, tol, nonfinite_ok=True)
+            if not ok:
+                fell_back = True
+                self.prob.set_fail_mask(mask0)
+                self.prob.set("U", start)
+                its2, r, _ = self.prob.adr_solve(0, max_iters, tol)
+                its += its2
         self.prob.sync()
         sec = time.perf_counter() - t
         self.du_prev = self.prob.get("U") - u0
@@ -274,25 +382,62 @@ class Plate:
         return its, r, sec, fell_back, t_pred, used_net
 
 
+def race(a, guess_a, b, guess_b, tol, max_iters, chunk=256):
+    """ADR from guess_a on plate a and from guess_b on plate b at once (two
+    handles, two streams, two host threads; the C-ABI calls release the
+    interpreter lock).  Each runs chunks of `chunk` iterations and stops at
+    the first chunk boundary after either has reached tol.  Returns
+    (iterations, residual, b_won) of the run that finished first (a's when
+    neither reached tol)."""
+    import threading
+    done = threading.Event()
+    out = {}
+
+    def run(name, P, g):
+        P.prob.set("U", g)
+        its, r, ok = 0, float("inf"), False
+        while its < max_iters and not done.is_set():
+            m = min(chunk, max_iters - its)
+            it, r, ok = P.prob.adr_solve(its, m, tol)
+            its += it
+            if ok or it < m:
+                done.set()
+                break
+        out[name] = (its, r, ok)
+
+    tb = threading.Thread(target=run, args=("b", b, guess_b))
+    tb.start()
+    run("a", a, guess_a)
+    tb.join()
+    ia, ra, oka = out["a"]
+    ib, rb, okb = out["b"]
+    if okb and (not oka or ib < ia):
+        return ib, rb, True
+    return ia, ra, False
+
+
 def eval_specs(count, n_min=200, n_max=400, seed_base=0x5EED1004):
     return PB.plate_family(count, seed_base=seed_base, n_min=n_min, n_max=n_max, dx=DX)
 
 
-def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None):
+def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None, race_fallback=False):
     """Plain ADR, the warm start without the network (du_prev, uniform stretch)
     and with it, on the same plates and increments (three handles per plate,
-    each on its own load path).
+    each on its own load path; with race_fallback a fourth runs the network
+    path's plain-ADR fallback at the same time, see Plate.step_warm).  The
+    network run's seconds include its prediction and its fallback.
     Returns per-increment rows (spec, k, its_plain, its_extrap, its_net, t_plain,
     t_extrap, t_net, fallback_extrap, fallback_net, used_net)."""
     rows = []
     for s, (n, _, clen, ang) in enumerate(specs):
         plain, extrap, warm = Plate(n, clen, ang), Plate(n, clen, ang), Plate(n, clen, ang)
+        shadow = Plate(n, clen, ang) if race_fallback else None
         net.set_graph(warm.rp, warm.col)
         try:
             for k in range(n_inc):
                 ip, rp_, rprobe, tp = plain.step_plain(tol, probe=probe)
-                ie, re_, te, fe, _, _ = extrap.step_warm(net, rprobe, tol, probe=probe, use_net=False)
-                iw, rw, tw, fw, tpred, used = warm.step_warm(net, rprobe, tol, probe=probe, use_net=True)
+                ie, re_, te, fe, _, _ = extrap.step_warm(None, tol, use_net=False)
+                iw, rw, tw, fw, tpred, used = warm.step_warm(net, tol, use_net=True, fallback=shadow)
                 rows.append((s, k, ip, ie, iw, tp, te, tw, fe, fw, used))
                 if log:
                     log(f"[eval] plate {s} n={n} inc {k}: iterations plain {ip} no-net {ie} net {iw}  "
@@ -303,6 +448,8 @@ def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None):
             plain.close()
             extrap.close()
             warm.close()
+            if shadow is not None:
+                shadow.close()
     return rows
 
 
@@ -319,17 +466,19 @@ def main(argv=None):
     ap.add_argument("--plates", type=int, default=128)
     ap.add_argument("--first", type=int, default=0, help="index of the first plate of the seeded family")
     ap.add_argument("--increments", type=int, default=6)
+    ap.add_argument("--race", action="store_true", help="run the plain-ADR fallback beside the network's start")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                       "mpnn_warmstart.npy"))
     a = ap.parse_args(argv)
     specs = eval_specs(a.first + a.plates)[a.first:]
     with Mpnn() as net:
         net.set_params(np.load(a.weights))
-        rows = np.array(evaluate(net, specs, a.increments, log=lambda m: print(m, flush=True)), dtype=float)
+        rows = np.array(evaluate(net, specs, a.increments, log=lambda m: print(m, flush=True), race_fallback=a.race),
+                        dtype=float)
     first, later = rows[:, 1] == 0, rows[:, 1] >= 1
     tot = lambda sel, c: float(rows[sel, c].sum())
     every = rows[:, 1] >= 0
-    out = {"plates": a.plates, "first": a.first, "sides": [int(s[0]) for s in specs], "increments": a.increments,
+    out = {"plates": a.plates, "first": a.first, "race": a.race, "net_scale": NET_SCALE, "sides": [int(s[0]) for s in specs], "increments": a.increments,
            "iterations": {"plain": tot(every, 2), "warm_no_network": tot(every, 3), "warm_network": tot(every, 4)},
            "seconds": {"plain": tot(every, 5), "warm_no_network": tot(every, 6), "warm_network": tot(every, 7)},
            "first_increment_iterations": {"plain": tot(first, 2), "uniform_stretch": tot(first, 3),

@@ -144,3 +144,29 @@ def test_batch_group_barrier_timeout_is_reported():
         assert np.array_equal(its, its_ref) and np.array_equal(res, res_ref)
         for s in range(len(specs)):
             assert np.array_equal(b.get(s), u_ref[s]) and np.array_equal(b.csr(s)[2], fail_ref[s]), s
+
+
+def test_batch_warm_start_through_fracture():
+    # 2 um increments: bonds start to break at the sixth increment and the
+    # cracks run over the last three; the previous-increment start still
+    # saves iterations in total and every increment reaches tol.  Once bonds
+    # break the broken-bond sets of the two runs differ (the bonds that break
+    # during an ADR run depend on the path from its start), so only their
+    # sizes are compared.
+    p = PB.plate_params()
+    specs = PB.plate_family(12, n_min=24, n_max=48)
+    tot, broke = {}, {}
+    for mode in (0, 1):
+        with PB.Batch(p, specs) as b:
+            b.set_warm(mode)
+            its_all, nb = [], []
+            for inc in range(10):
+                its, res, _ = b.increment(2.0e-6, 1e-6, 40000, block=1024)
+                assert np.all(res <= 1e-6), (mode, inc, res)
+                its_all.append(int(its.sum()))
+                nb.append(sum(int((b.csr(s_)[2] == 0).sum()) for s_ in range(len(specs))))
+            tot[mode], broke[mode] = np.array(its_all), np.array(nb)
+            print("mode", mode, "iterations per increment", its_all, "broken", nb, flush=True)
+    assert broke[0][-1] > 2 * broke[0][0] and broke[1][-1] > 2 * broke[1][0]
+    assert tot[1].sum() < 0.8 * tot[0].sum()
+    assert abs(broke[1][-1] - broke[0][-1]) < 0.15 * broke[0][-1]

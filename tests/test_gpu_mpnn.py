@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from pd_gpu1_amd import _lib
-from pd_gpu1_amd.mpnn import Mpnn
+from pd_gpu1_amd.mpnn import N_FEAT, Mpnn
 from oracle import mpnn_ref as R
 from oracle import oracle as O
 
@@ -23,7 +23,7 @@ def _graph(n=14, dx=1e-3):
     rng = np.random.default_rng(3)
     intact = (rng.uniform(size=len(col)) > 0.1).astype(np.float64)
     ef = np.concatenate([dvec, intact[:, None]], axis=1)
-    xf = np.concatenate([rng.uniform(-1, 1, (len(x), 7)), np.ones((len(x), 1))], axis=1)
+    xf = np.concatenate([rng.uniform(-1, 1, (len(x), N_FEAT - 1)), np.ones((len(x), 1))], axis=1)
     return rp, col, ef, xf
 
 

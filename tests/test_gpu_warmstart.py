@@ -38,7 +38,7 @@ def test_warm_and_plain_reach_the_same_state():
             net.set_graph(warm.rp, warm.col)
             for _ in range(3):
                 _, r_p, r_probe, _ = plain.step_plain()
-                _, r_w, _, _, _, _ = warm.step_warm(net, r_probe, use_net=True)
+                _, r_w, _, _, _, _ = warm.step_warm(net, use_net=True)
                 assert r_p <= 1e-6 and r_w <= 1e-6
         up, uw = plain.prob.get("U"), warm.prob.get("U")
         assert np.max(np.abs(up - uw)) < 1e-3 * np.max(np.abs(up))
@@ -46,3 +46,74 @@ def test_warm_and_plain_reach_the_same_state():
     finally:
         plain.close()
         warm.close()
+
+
+def test_two_handles_run_from_two_threads():
+    # each handle keeps its copies on its own stream, so one thread may
+    # capture a handle's step graph while another copies into its handle;
+    # the results equal those of the same calls made one after the other
+    import threading
+    n = 40
+    plates = [W.Plate(n, 0.3 * n * W.DX, 0.1 * k) for k in (1, 2, 1, 2)]
+    tols = (1e-6, 1e-7)
+
+    def load(P, tol, out):
+        for _ in range(3):
+            u = P.prob.get("U")
+            u[1, P.top] += W.DU
+            u[1, P.bot] -= W.DU
+            P.prob.set("U", u)
+            its, r, ok = P.prob.adr_solve(0, 200000, tol)
+            assert ok, (its, r)
+            out.append(its)
+
+    try:
+        seq = [[], []]
+        for k in range(2):
+            load(plates[k], tols[k], seq[k])
+        par = [[], []]
+        th = [threading.Thread(target=load, args=(plates[2 + k], tols[k], par[k])) for k in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert par == seq
+        for k in range(2):
+            assert np.array_equal(plates[k].prob.get("U"), plates[2 + k].prob.get("U"))
+            assert np.array_equal(plates[k].prob.fail_mask(), plates[2 + k].prob.fail_mask())
+    finally:
+        for P in plates:
+            P.close()
+
+
+def test_shipped_network_on_a_large_unseen_plate():
+    # the shipped weights on the first plate of the unseen evaluation family
+    # (315 x 315 free particles, 101,115 with the strips): the network's start
+    # reaches tol in fewer iterations than the uniform-stretch start, and both
+    # end in the state plain ADR reaches, with the same broken bonds
+    import os
+    wpath = os.path.join(os.path.dirname(W.__file__), "data", "mpnn_warmstart.npy")
+    n, _, clen, ang = W.eval_specs(1)[0]
+    assert n == 315
+    plates = [W.Plate(n, clen, ang) for _ in range(3)]
+    plain, extrap, warm = plates
+    try:
+        with Mpnn() as net:
+            net.set_params(np.load(wpath))
+            net.set_graph(warm.rp, warm.col)
+            ip, rp_, _, _ = plain.step_plain()
+            ie, re_, _, fe, _, used_e = extrap.step_warm(None, use_net=False)
+            iw, rw, _, fw, _, used_w = warm.step_warm(net, use_net=True)
+        print("iterations plain", ip, "uniform stretch", ie, "network", iw)
+        assert used_w and not used_e and not fw and not fe
+        assert max(rp_, re_, rw) <= 1e-6
+        assert iw < ie and iw < ip
+        up = plain.prob.get("U")
+        for P in (extrap, warm):
+            d = np.max(np.abs(P.prob.get("U") - up)) / np.max(np.abs(up))
+            print("max |u - u_plain| / max |u_plain|", d)
+            assert d < 1e-3
+            assert np.array_equal(P.prob.fail_mask(), plain.prob.fail_mask())
+    finally:
+        for P in plates:
+            P.close()

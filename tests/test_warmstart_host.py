@@ -15,8 +15,10 @@ def test_node_and_edge_features_layout():
     dp = np.array([[1e-6, 0.0, 0.0], [0.0, -1e-6, 0.0]])
     top = np.array([False, True, False])
     bot = np.array([False, False, True])
-    f = W.node_features(pos, dp, np.array([0.0, 0.5, 1.0]), top, bot)
-    assert f.shape == (3, 8) and f.dtype == np.float32
+    crack = np.arange(24, dtype=np.float64).reshape(8, 3)
+    f = W.node_features(pos, dp, np.array([0.0, 0.5, 1.0]), top, bot, crack)
+    assert f.shape == (3, 16) and f.dtype == np.float32
+    np.testing.assert_allclose(f[:, 8:16], crack.T)
     np.testing.assert_allclose(f[:, 0:2], pos.T)
     np.testing.assert_allclose(f[:, 2], [1.0, 0.0, 0.0])
     np.testing.assert_allclose(f[:, 3], [0.0, -1.0, 0.0])
@@ -52,7 +54,7 @@ def test_samples_are_first_increments_and_post_break_increments():
     broken = intact.copy()
     broken[3] = 0
     # a bond breaks during the second increment (between states 1 and 2)
-    rec = dict(n=n, n_int=n_int, rp=rp, col=col, x=x, y=y, top=top, bot=bot, u=us,
+    rec = dict(n=n, n_int=n_int, rp=rp, col=col, x=x, y=y, top=top, bot=bot, u=us, clen=1e-3, angle=0.3,
                mask=[intact, intact, broken, broken, broken])
     out = list(W.samples(rec, delta=1.5e-3))
     assert len(out) == 2  # increment 1 (no history) and increment 3 (after the break)
@@ -78,3 +80,89 @@ def test_base_increment_moves_strips_by_du():
     order = np.argsort(y, kind="stable")
     assert np.all(np.diff(b[1, order]) >= 0.0)
     np.testing.assert_allclose(b[1, :n_int].mean(), 0.0, atol=1e-20)
+
+
+def test_crack_features_follow_the_crack_frame():
+    half = 2.0e-3
+    clen, ang = 2.0e-3, 0.4  # crack half-length a = 0.5 half sides
+    c, s = np.cos(ang), np.sin(ang)
+    # along the crack: its centre, a tip, beyond the tip; across it: just above and below the centre
+    along = np.array([0.0, 0.5, 1.0])
+    pos = np.concatenate([np.stack([along * c, along * s]), np.array([[-s * 0.05, s * 0.05], [c * 0.05, -c * 0.05]])],
+                         axis=1)
+    f = W.crack_features(pos, clen, ang, half)
+    np.testing.assert_allclose(f[0, :3], [0.0, 1.0 / 3.0, 2.0 / 3.0], atol=1e-12)   # s / 3
+    np.testing.assert_allclose(f[1, 3:], [0.1 / 3.0, -0.1 / 3.0], atol=1e-12)      # t / 3
+    np.testing.assert_allclose(f[2, 3:], [np.sqrt(1.0), -np.sqrt(1.0)], atol=1e-12)  # opening: +1 above, -1 below
+    assert f[2, 2] == 0.0                                                          # beyond the tip
+    np.testing.assert_allclose(f[3], 0.5)
+    np.testing.assert_allclose(f[4:6, 0], [c, s])
+    np.testing.assert_allclose(f[6, 1], 1.0)                                       # at the tip
+    np.testing.assert_allclose(f[7, 0], 1.0)                                       # at the centre
+    assert not W.crack_features(pos, 0.0, ang, half).any()
+
+
+def test_mirror_maps_are_point_reflections():
+    from types import SimpleNamespace
+    n = 6
+    x, y, n_int, top, bot = W.plate_layout(n)
+    P = np.stack([x, y], 1)
+    d2 = ((P[:, None] - P[None]) ** 2).sum(-1)
+    N = len(x)
+    rows = [np.nonzero((d2[i] < (3.015 * W.DX) ** 2) & (np.arange(N) != i))[0] for i in range(N)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    plate = SimpleNamespace(x=x, y=y, n_int=n_int, rp=rp, col=col, _mir=None)
+    mir, bmir = W.Plate.mirror_maps(plate)
+    cx, cy = x[:n_int].mean(), y[:n_int].mean()
+    np.testing.assert_allclose(x[mir] - cx, -(x - cx), atol=1e-15)
+    np.testing.assert_allclose(y[mir] - cy, -(y - cy), atol=1e-15)
+    assert np.array_equal(mir[mir], np.arange(N)) and np.all(mir[:n_int] < n_int)
+    assert np.array_equal(mir[top], np.sort(np.nonzero(bot)[0])[::-1]) or set(mir[top]) == set(np.nonzero(bot)[0])
+    r = np.repeat(np.arange(N), np.diff(rp))
+    assert np.array_equal(r[bmir], mir[r]) and np.array_equal(col[bmir], mir[col])
+    assert np.array_equal(bmir[bmir], np.arange(len(col)))
+
+
+class _FakeProb:
+    """Stands in for a handle: converges after `need` iterations."""
+
+    def __init__(self, need, delay=0.0):
+        self.need, self.delay, self.calls, self.u = need, delay, [], None
+
+    def set(self, name, u):
+        self.u = u
+
+    def adr_solve(self, ct0, n, tol):
+        import time
+        time.sleep(self.delay)
+        self.calls.append((ct0, n))
+        left = self.need - ct0
+        if left <= n:
+            return left, tol * 0.5, True
+        return n, 1.0, False
+
+
+class _FakePlate:
+    def __init__(self, need, delay=0.0):
+        self.prob = _FakeProb(need, delay)
+
+
+def test_race_returns_the_first_run_to_converge():
+    a, b = _FakePlate(700, 0.002), _FakePlate(300, 0.002)
+    its, r, b_won = W.race(a, "ga", b, "gb", 1e-6, 10000, chunk=100)
+    assert b_won and its == 300 and r < 1e-6
+    assert a.prob.u == "ga" and b.prob.u == "gb"
+    # chunks continue the iteration count; the loser stopped early
+    assert [c[0] for c in b.prob.calls] == [0, 100, 200]
+    assert len(a.prob.calls) < 7
+
+
+def test_race_keeps_the_guess_when_it_wins_or_both_fail():
+    a, b = _FakePlate(250), _FakePlate(900, 0.01)
+    its, r, b_won = W.race(a, "ga", b, "gb", 1e-6, 10000, chunk=100)
+    assert not b_won and its == 250
+    a, b = _FakePlate(10**6), _FakePlate(10**6)
+    its, r, b_won = W.race(a, "ga", b, "gb", 1e-6, 500, chunk=128)
+    assert not b_won and its == 500
+    assert [c[1] for c in a.prob.calls] == [128, 128, 128, 116]
