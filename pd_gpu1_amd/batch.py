@@ -129,6 +129,27 @@ class Batch:
         """1: start each increment from u + the previous increment (0: plain)."""
         self._c(lib().pd_batch_set_warm(self._h, mode))
 
+    def set_tail(self, on):
+        """1: finish an increment's slowest specimens in further launches with
+        more workgroups each once the device is mostly idle (0: one launch)."""
+        self._c(lib().pd_batch_set_tail(self._h, int(on)))
+
+    def tail_parts(self):
+        w = C.c_int()
+        self._c(lib().pd_batch_get_tail_parts(self._h, C.byref(w)))
+        return w.value
+
+    def set_separation(self, ratio):
+        """End a specimen's increment once its strip reaction falls below
+        ratio x its last converged reaction (0: off)."""
+        self._c(lib().pd_batch_set_separation(self._h, ratio))
+
+    def retired(self):
+        """Per specimen: 0 active, 1 retired unconverged, 2 retired separated."""
+        out = np.zeros(self.nspec, np.int32)
+        self._c(lib().pd_batch_get_retired(self._h, out.ctypes.data_as(_lib.I32)))
+        return out
+
     def set_retire(self, on):
         """1: skip specimens from the increment after one that ended unconverged."""
         self._c(lib().pd_batch_set_retire(self._h, on))
@@ -184,6 +205,11 @@ def main(argv=None):
     ap.add_argument("--parts", type=int, default=0, help="workgroups per specimen (0: sized from the work)")
     ap.add_argument("--warm", type=int, default=0)
     ap.add_argument("--retire", type=int, default=1)
+    ap.add_argument("--tail", type=int, default=1,
+                    help="1: finish the slowest specimens in further launches with more workgroups (0: one launch)")
+    ap.add_argument("--separation", type=float, default=0.0,
+                    help="end a specimen's increment once its strip reaction falls below this fraction of its last "
+                         "converged reaction (0: off)")
     a = ap.parse_args(argv)
     specs = plate_family(a.plates)
     rows = []
@@ -193,6 +219,8 @@ def main(argv=None):
         b.set_parts(a.parts)
         b.set_warm(a.warm)
         b.set_retire(a.retire)
+        b.set_tail(a.tail)
+        b.set_separation(a.separation)
         n_part, n_bonds, _ = b.info()
         print(f"plates {a.plates} particles {n_part} bonds {n_bonds}", flush=True)
         for k in range(a.increments):
@@ -201,11 +229,13 @@ def main(argv=None):
             rows.append({"increment": k, "ms": round(ms, 1), "active": int(act.sum()), "parts": b.parts_used(),
                          "iterations_median": int(np.median(its[act])) if act.any() else 0,
                          "iterations_max": int(its.max()), "unconverged": int(((res > a.tol) & act).sum()),
-                         "specimen_iterations": int(its.sum())})
+                         "specimen_iterations": int(its.sum()), "tail_parts": b.tail_parts(),
+                         "separated": int((b.retired() == 2).sum())})
             print("increment %(increment)d: %(ms).1f ms, %(active)d active, iterations median "
-                  "%(iterations_median)d max %(iterations_max)d, unconverged %(unconverged)d" % rows[-1], flush=True)
+                  "%(iterations_median)d max %(iterations_max)d, unconverged %(unconverged)d, separated so far "
+                  "%(separated)d, tail launch parts %(tail_parts)d" % rows[-1], flush=True)
     print(json.dumps({"plates": a.plates, "particles": n_part, "bonds": n_bonds, "block": a.block,
-                      "group": a.group, "warm": a.warm, "total_s": sum(r["ms"] for r in rows) / 1e3,
+                      "group": a.group, "warm": a.warm, "tail": a.tail, "separation": a.separation, "total_s": sum(r["ms"] for r in rows) / 1e3,
                       "wall_s": time.perf_counter() - t0, "increments": rows}), flush=True)
 
 

@@ -122,7 +122,8 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
 {
     __shared__ double sh[4][BT / 64];
     __shared__ int spec_sh, ok_sh;
-    __shared__ double c_sh, r_sh;
+    __shared__ double c_sh, r_sh, rd_sh;
+    __shared__ int sep_sh, pause_sh;
     __shared__ double q_sh[kMaxParts][4];
     const int tid = threadIdx.x;
     const int W = gr.parts;                       // workgroups per specimen
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
     for (;;) {
         if (tid == 0 && part == 0) {
             const int t = (int)atomicAdd(queue, 1u);
-            const int q = t < B.nspec ? B.order[t] : B.nspec;
+            const int q = t < B.nqueue ? B.order[t] : B.nspec;
             if (W == 1)
                 spec_sh = q;
             else
@@ -153,7 +154,11 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         PD_GROUP_SYNC();  // every part has read s before part 0 takes the next ticket
         if (s >= B.nspec) break;
         if (B.retired && B.retired[s]) {
-            if (tid == 0 && part == 0) iters_out[s] = 0;
+            if (tid == 0 && part == 0) {
+                iters_out[s] = 0;
+                if (B.paused) B.paused[s] = 0;
+                if (B.active) atomicSub(B.active, 1);
+            }
             continue;  // uniform: every thread of the group read the same s
         }
         const BatchSpec d = B.spec[s];
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         const double mass = B.mass, dt = B.dt, s0 = B.s0;
         double *DUPX = B.dupx + d.p_off, *DUPY = B.dupy + d.p_off;
         double2 *P = B.pos + d.p_off;
-        if (B.warm) {
+        if (B.warm && !B.resume) {
             // start from u + the previous increment; keep the start state
             for (int i = gt; i < Ni; i += GT) {
                 const double sx = UX[i], sy = UY[i];
@@ -183,13 +188,25 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
             }
         }
         // displacement control of the strips (Solve_Plate_Crack_Grow.cu:20-27 layout)
-        for (int k = gt; k < d.n_top + d.n_bot; k += GT) UY[Ni + k] += k < d.n_top ? du : -du;
+        if (!B.resume)
+            for (int k = gt; k < d.n_top + d.n_bot; k += GT) UY[Ni + k] += k < d.n_top ? du : -du;
         PD_GROUP_SYNC();
         for (int i = gt; i < N; i += GT) P[i] = make_double2(X[i] + UX[i], Y[i] + UY[i]);
         PD_GROUP_SYNC();
-        int it = 0;
-        double r = 0.0;
+        // A resumed specimen continues at the iteration it paused at: its
+        // state (U, VHO, FO, P) is what the paused launch left, so the
+        // iterations are those of one uninterrupted run.  A pause is decided
+        // during an iteration and taken at the top of the next, before its
+        // force evaluation.
+        int it = B.resume ? uniform_int(B.it0[s]) : 0;
+        double r = B.resume ? resid_out[s] : 0.0;
+        bool paused = false, separated = false;
+        int pause_next = 0;
         for (;;) {
+            if (pause_next) {
+                paused = true;
+                break;
+            }
             // PMB force, every row (Force_Bond.cu:169).  A part computes the
             // rows of the particles on its own damping-tree lanes (i mod BT
             // in [part*L, part*L + L)), so the damping sums below need only
@@ -278,14 +295,24 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
             // lanes from L on add zeros, so with W > 1 the workgroup's tree
             // returns the root of this part's subtree exactly
             block_canon4<BT>(p, sh, tot);
+            // pause when every specimen has been handed out and the device
+            // runs at most pause_below of them (part 0 decides for the group)
+            int pf = 0;
+            if (B.pause_below > 0 && tid == 0 && part == 0)
+                pf = __hip_atomic_load(queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned)B.nqueue &&
+                     __hip_atomic_load(B.active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= B.pause_below;
             if (W > 1) {
-                if (tid == 0)
-                    for (int c = 0; c < 4; ++c) gr.sum[((size_t)grp * W + part) * 4 + c] = tot[c];
-                PD_GROUP_SYNC();
                 if (tid == 0) {
+                    for (int c = 0; c < 4; ++c) gr.sum[((size_t)grp * W + part) * 4 + c] = tot[c];
+                    if (part == 0) gr.flag[grp] = pf;
+                }
+                PD_GROUP_SYNC();
+                if (tid < W)
+                    for (int c = 0; c < 4; ++c) q_sh[tid][c] = gr.sum[((size_t)grp * W + tid) * 4 + c];
+                __syncthreads();
+                if (tid == 0) {
+                    pf = gr.flag[grp];
                     // the remaining levels of the tree, over the W subtree roots
-                    for (int w = 0; w < W; ++w)
-                        for (int c = 0; c < 4; ++c) q_sh[w][c] = gr.sum[((size_t)grp * W + w) * 4 + c];
                     for (int off = 1; off < W; off <<= 1)
                         for (int w = 0; w < W; w += 2 * off)
                             for (int c = 0; c < 4; ++c) q_sh[w][c] = q_sh[w][c] + q_sh[w + off][c];
@@ -304,11 +331,21 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                 }
                 if (cc > 2.0) cc = 1.9;
                 c_sh = cc;
+                pause_sh = pf;
                 r_sh = tot[3] > 0.0 ? sqrt(tot[2]) / sqrt(tot[3]) : (tot[2] > 0.0 ? INFINITY : 0.0);
+                rd_sh = sqrt(tot[3]);
+                // separation: the strips' reaction has collapsed against the
+                // last converged increment's (the plate no longer carries load)
+                sep_sh = B.sep > 0.0 && B.react[s] > 0.0 && sqrt(tot[3]) < B.sep * B.react[s];
             }
             __syncthreads();
             r = uniform_double(r_sh);
             if (r <= tol || it >= max_iters || !(r == r)) break;
+            if (uniform_int(sep_sh)) {
+                separated = true;
+                break;
+            }
+            pause_next = uniform_int(pause_sh);
             const double cc = uniform_double(c_sh);
             for (int i = gt; i < Ni; i += GT) {
                 double h0, h1;
@@ -336,7 +373,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
             PD_GROUP_SYNC();
             ++it;
         }
-        if (B.warm) {
+        if (B.warm && !paused) {
             for (int i = gt; i < Ni; i += GT) {
                 DUPX[i] = UX[i] - DUPX[i];
                 DUPY[i] = UY[i] - DUPY[i];
@@ -345,7 +382,12 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         if (tid == 0 && part == 0) {
             iters_out[s] = it;
             resid_out[s] = r;
-            if (B.retired && !(r <= tol)) B.retired[s] = 1;
+            if (B.paused) B.paused[s] = paused ? 1 : 0;
+            if (B.active) atomicSub(B.active, 1);
+            if (!paused) {
+                if (B.retired && !(r <= tol)) B.retired[s] = separated ? 2 : 1;
+                if (r <= tol && B.react) B.react[s] = rd_sh;
+            }
         }
         PD_GROUP_SYNC();
     }
@@ -406,8 +448,9 @@ hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, cons
                             const BatchGroups &gr)
 {
     // W must divide the grid and the damping tree (block/64 waves)
-    if (gr.parts < 1 || gr.parts > kMaxParts || (gr.parts & (gr.parts - 1)) || block / 64 < gr.parts ||
-        grid % gr.parts || (gr.parts > 1 && (!gr.bar || !gr.spec || !gr.sum || !gr.abort)))
+    if (gr.parts < 1 || gr.parts > kMaxParts || (gr.parts & (gr.parts - 1)) || block / kMinPartLanes < gr.parts ||
+        grid % gr.parts || (gr.parts > 1 && (!gr.bar || !gr.spec || !gr.sum || !gr.abort || !gr.flag)) ||
+        (B.pause_below > 0 && (!B.active || !B.paused)))
         return hipErrorInvalidValue;
     switch (block) {
     case 256: return batch_g<256>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);

@@ -19,7 +19,17 @@ struct BatchView {
     double *dupx, *dupy;  // previous increment of u (warm start), then the start state
     double2 *pos;         // per particle (x + ux, y + uy): the one 16-byte gather of a bond
     int warm;             // 1: start each increment from u + previous increment
-    int32_t *retired;     // non-null: skip specimens marked 1, mark those that end unconverged
+    int32_t *retired;     // non-null: skip specimens marked non-zero, mark those that end unconverged
+                          // (1) or separated (2)
+    int nqueue;           // queue tickets: order[0 .. nqueue-1]
+    int resume;           // 1: the queued specimens continue an increment paused at iteration it0[s]
+    const int32_t *it0;   // resume: the iteration each specimen paused at
+    int pause_below;      // > 0: once the queue is empty and at most this many specimens are still
+                          // running, they pause at their next iteration (0: never)
+    int *active;          // specimens of this launch not yet ended or paused
+    int32_t *paused;      // [nspec] 1 for a specimen this launch paused
+    double sep;           // > 0: a specimen whose strip reaction falls below sep * react[s] has separated
+    double *react;        // [nspec] strip reaction |F| at the specimen's last converged increment (0: none)
     const int32_t *row_ptr, *col;
     const double2 *kd;    // per bond {kappa, |xi|}
     uint32_t *fail;
@@ -28,12 +38,18 @@ struct BatchView {
 
 // Several workgroups per specimen (k_batch_adr): workgroups per group, and
 // per group an arrival counter, the specimen slot and the parts' damping sums.
-constexpr int kMaxParts = 16;
+// A part owns block/parts lanes of the damping tree (a power of two, at least
+// kMinPartLanes); the first launch of an increment uses at most 16 parts,
+// the tail launches up to kMaxParts.
+constexpr int kMaxParts = 64;
+constexpr int kMaxFirstParts = 16;
+constexpr int kMinPartLanes = 16;
 struct BatchGroups {
-    int parts;      // workgroups per specimen: a power of two <= min(16, block/64)
+    int parts;      // workgroups per specimen: a power of two <= min(kMaxParts, block/kMinPartLanes)
     unsigned *bar;  // [groups] arrival counters, zero at launch
     int *spec;      // [groups] the group's current specimen
     double *sum;    // [groups][parts][4] subtree roots of the damping tree
+    int *flag;      // [groups] part 0's pause decision, read by every part after the group barrier
     int *abort;     // set when a group barrier times out
     uint64_t spin_ticks;  // longest wait at a group barrier, in wall-clock ticks
 };

@@ -30,21 +30,30 @@ struct pd_batch {
     unsigned *queue = nullptr;
     int32_t *d_iters = nullptr;
     double *d_resid = nullptr;
-    int32_t *d_retired = nullptr; // per specimen: 1 once an increment ended unconverged (with retire on)
+    int32_t *d_retired = nullptr; // per specimen: 1 once an increment ended unconverged, 2 separated (retire on)
+    double *d_react = nullptr;    // per specimen: strip reaction at its last converged increment
+    int32_t *d_order2 = nullptr;  // queue of the paused specimens a tail launch continues
+    int tail = 1;                 // 1: tail launches (pd_batch_set_tail)
+    double sep = 0.0;             // separation ratio (0: off)
+    int *d_active = nullptr;      // specimens of a launch not yet ended or paused
+    int32_t *d_paused = nullptr;  // [nspec] specimens a launch paused
     int retire = 0;
     int last_block = 1024;
     int group = 1;                // lanes per row of the force phase (1 = thread per particle)
     int parts = 0;                // workgroups per specimen (0 = sized from the batch's work)
     int last_parts = 1;           // the value the last increment ran with
+    int last_tail_parts = 0;      // workgroups per specimen of its second launch (0: none)
     int groups_cap = 0;           // groups the group buffers hold
     unsigned *g_bar = nullptr;    // [groups_cap] barrier arrival counters
     int *g_spec = nullptr;        // [groups_cap] specimen of each group
     double *g_sum = nullptr;      // [groups_cap][kMaxParts][4] damping subtree roots
     int *g_abort = nullptr;       // a group barrier timed out
+    int *g_flag = nullptr;        // [groups_cap] pause decision of each group
     double *snap = nullptr;       // state before a multi-workgroup increment: 14 particle fields,
     double2 *snap_pos = nullptr;  // positions, fail words and retired flags, restored when
     uint32_t *snap_fail = nullptr;  // the increment is abandoned at a barrier timeout
     int32_t *snap_ret = nullptr;
+    double *snap_react = nullptr;
     double barrier_timeout = 20.0;  // seconds a workgroup waits at a group barrier
     int warm = 0;                 // 1: increments start from u + the previous increment
     BatchView view() const
@@ -62,6 +71,14 @@ struct pd_batch {
         v.pos = pos;
         v.warm = warm;
         v.retired = retire ? d_retired : nullptr;
+        v.nqueue = (int)spec.size();
+        v.resume = 0;
+        v.it0 = nullptr;
+        v.pause_below = 0;
+        v.active = d_active;
+        v.paused = d_paused;
+        v.sep = sep;
+        v.react = d_react;
         v.row_ptr = row_ptr; v.col = col; v.kd = kd; v.fail = fail;
         v.mass = P.mass; v.dt = P.dt; v.s0 = P.s0;
         return v;
@@ -80,7 +97,9 @@ struct pd_batch {
 static void batch_free(pd_batch *b)
 {
     void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->fail, b->queue, b->d_iters,
-                    b->d_resid, b->d_retired, b->g_bar, b->g_spec, b->g_sum, b->g_abort, b->snap, b->snap_pos,
+                    b->d_resid, b->d_retired, b->d_react, b->d_order2, b->snap_react, b->d_active, b->d_paused,
+                    b->g_flag, b->g_bar, b->g_spec,
+                    b->g_sum, b->g_abort, b->snap, b->snap_pos,
                     b->snap_fail, b->snap_ret};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -158,6 +177,11 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
     if (!rc) rc = alloc((void **)&b->d_resid, sizeof(double) * nspec);
     if (!rc) rc = alloc((void **)&b->d_retired, sizeof(int32_t) * nspec);
     if (!rc && fill_on(b->stream, b->d_retired, 0, sizeof(int32_t) * nspec) != hipSuccess) rc = PD_E_HIP;
+    if (!rc) rc = alloc((void **)&b->d_react, sizeof(double) * nspec);
+    if (!rc && fill_on(b->stream, b->d_react, 0, sizeof(double) * nspec) != hipSuccess) rc = PD_E_HIP;
+    if (!rc) rc = alloc((void **)&b->d_order2, sizeof(int32_t) * nspec);
+    if (!rc) rc = alloc((void **)&b->d_active, sizeof(int));
+    if (!rc) rc = alloc((void **)&b->d_paused, sizeof(int32_t) * nspec);
     if (!rc && fill_on(b->stream, b->buf, 0, sizeof(double) * 16 * b->n_total) != hipSuccess) rc = PD_E_HIP;
     for (int s = 0; s < nspec && !rc; s++) {
         pd_handle *h = hs[s];
@@ -240,11 +264,66 @@ int pd_batch_info(pd_batch *b, int64_t *n_particles, int64_t *n_bonds, int *nspe
     return PD_OK;
 }
 
+// Group buffers for `groups` groups of workgroups (barrier counters,
+// specimen slots, damping subtree roots, abort flag).
+static int ensure_groups(pd_batch *b, int groups)
+{
+    if (groups <= b->groups_cap) return PD_OK;
+    B_TRY(hipStreamSynchronize(b->stream));
+    void *old[] = {b->g_bar, b->g_spec, b->g_sum, b->g_abort, b->g_flag};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    b->g_bar = nullptr; b->g_spec = nullptr; b->g_sum = nullptr; b->g_abort = nullptr; b->g_flag = nullptr;
+    b->groups_cap = 0;
+    B_TRY(hipMalloc((void **)&b->g_bar, sizeof(unsigned) * groups));
+    B_TRY(hipMalloc((void **)&b->g_spec, sizeof(int) * groups));
+    B_TRY(hipMalloc((void **)&b->g_sum, sizeof(double) * 4 * kMaxParts * groups));
+    B_TRY(hipMalloc((void **)&b->g_abort, sizeof(int)));
+    B_TRY(hipMalloc((void **)&b->g_flag, sizeof(int) * groups));
+    b->groups_cap = groups;
+    return PD_OK;
+}
+
+// Snapshot (save = true) or restore of the state an increment starts from:
+// particle fields 2..15, positions, fail words, retired flags and reactions.
+static int batch_snapshot(pd_batch *b, bool save)
+{
+    const size_t nfield = (size_t)b->n_total * 14, nwords = (size_t)(b->nb_total + 31) / 32;
+    const size_t ns = b->spec.size();
+    if (!b->snap) {
+        B_TRY(hipMalloc((void **)&b->snap, sizeof(double) * nfield));
+        B_TRY(hipMalloc((void **)&b->snap_pos, sizeof(double2) * b->n_total));
+        B_TRY(hipMalloc((void **)&b->snap_fail, sizeof(uint32_t) * nwords));
+        B_TRY(hipMalloc((void **)&b->snap_ret, sizeof(int32_t) * ns));
+        B_TRY(hipMalloc((void **)&b->snap_react, sizeof(double) * ns));
+    }
+    double *u0 = b->buf + (size_t)2 * b->n_total;  // fields 2..15 are contiguous
+    struct { void *live, *copy; size_t bytes; } parts[] = {
+        {u0, b->snap, sizeof(double) * nfield},
+        {b->pos, b->snap_pos, sizeof(double2) * b->n_total},
+        {b->fail, b->snap_fail, sizeof(uint32_t) * nwords},
+        {b->d_retired, b->snap_ret, sizeof(int32_t) * ns},
+        {b->d_react, b->snap_react, sizeof(double) * ns}};
+    for (auto &q : parts)
+        B_TRY(hipMemcpyAsync(save ? q.copy : q.live, save ? q.live : q.copy, q.bytes, hipMemcpyDeviceToDevice,
+                             b->stream));
+    if (!save) B_TRY(hipStreamSynchronize(b->stream));
+    return PD_OK;
+}
+
 int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int block, int grid,
                        int32_t *iters_out, double *resid_out, double *ms_out)
 {
     // One load increment for every specimen: strips move by +du / -du, then
     // ADR iterates each specimen until its residual <= tol (or max_iters).
+    //
+    // With the tail launch on (pd_batch_set_tail), the increment runs in up
+    // to two persistent launches: the first pauses every specimen that
+    // reaches tail x the previous increment's median iteration count, and
+    // the second continues only those, with more workgroups each (the
+    // slowest specimens otherwise run on a few workgroups while the rest of
+    // the device idles).  A paused specimen resumes at the iteration it
+    // paused at, so results are identical to one launch.
     if (!b || max_iters < 0 || !(tol >= 0.0)) return PD_E_INVALID;
     if (block != 256 && block != 512 && block != 1024) {
         b->err = "block must be 256, 512 or 1024";
@@ -257,7 +336,8 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     B_TRY(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, b->device));
     const int capacity = dev_cu * batch_resident_per_cu(block, b->group);
     int W = b->parts;
-    const int wmax = std::min(kMaxParts, block / 64);
+    const int wmax = std::min(kMaxFirstParts, block / 64);
+    const int wtail = std::min(kMaxParts, block / kMinPartLanes);
     if (W == 0) {
         std::vector<int32_t> ret((size_t)nspec, 0);
         if (b->retire)
@@ -284,78 +364,136 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     B_TRY(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, b->device));
     const double ticks = b->barrier_timeout * (clock_khz > 0 ? clock_khz : 100000) * 1e3;
     const uint64_t spin = ticks < 1.0 ? 1ull : ticks > 1e18 ? (uint64_t)1e18 : (uint64_t)ticks;
-    BatchGroups gr{W, nullptr, nullptr, nullptr, nullptr, spin};
-    if (W > 1) {
-        if (groups > b->groups_cap) {
-            void *old[] = {b->g_bar, b->g_spec, b->g_sum, b->g_abort};
-            for (void *q : old)
-                if (q) (void)hipFree(q);
-            b->g_bar = nullptr; b->g_spec = nullptr; b->g_sum = nullptr; b->g_abort = nullptr;
-            b->groups_cap = 0;
-            B_TRY(hipMalloc((void **)&b->g_bar, sizeof(unsigned) * groups));
-            B_TRY(hipMalloc((void **)&b->g_spec, sizeof(int) * groups));
-            B_TRY(hipMalloc((void **)&b->g_sum, sizeof(double) * 4 * kMaxParts * groups));
-            B_TRY(hipMalloc((void **)&b->g_abort, sizeof(int)));
-            b->groups_cap = groups;
-        }
-        gr = BatchGroups{W, b->g_bar, b->g_spec, b->g_sum, b->g_abort, spin};
-        B_TRY(hipMemsetAsync(b->g_bar, 0, sizeof(unsigned) * groups, b->stream));
-        B_TRY(hipMemsetAsync(b->g_abort, 0, sizeof(int), b->stream));
+    const bool tail = b->tail && grid <= 0 && W < wtail;
+    // With several workgroups per specimen (or further launches) the
+    // increment can be abandoned part-way at a barrier timeout (strips moved,
+    // some specimens relaxed, others not): keep the state it starts from so
+    // that it can be put back.
+    const bool snapshot = W > 1 || tail;
+    if (snapshot) {
+        int rc = batch_snapshot(b, true);
+        if (rc) return rc;
     }
-    // With several workgroups per specimen a barrier timeout abandons the
-    // increment part-way (strips moved, some specimens relaxed, others not):
-    // keep the state it starts from so that it can be put back.
-    const size_t nfield = (size_t)b->n_total * 14, nwords = (size_t)(b->nb_total + 31) / 32;
-    const size_t nspec_sz = b->spec.size();
-    if (W > 1) {
-        if (!b->snap) {
-            B_TRY(hipMalloc((void **)&b->snap, sizeof(double) * nfield));
-            B_TRY(hipMalloc((void **)&b->snap_pos, sizeof(double2) * b->n_total));
-            B_TRY(hipMalloc((void **)&b->snap_fail, sizeof(uint32_t) * nwords));
-            B_TRY(hipMalloc((void **)&b->snap_ret, sizeof(int32_t) * nspec_sz));
+    // one persistent launch of `gw` groups of `w` workgroups over the view's queue
+    auto launch = [&](const BatchView &v, int w, int gw) -> int {
+        BatchGroups gr{w, nullptr, nullptr, nullptr, nullptr, nullptr, spin};
+        if (w > 1) {
+            int rc = ensure_groups(b, gw);
+            if (rc) return rc;
+            gr = BatchGroups{w, b->g_bar, b->g_spec, b->g_sum, b->g_flag, b->g_abort, spin};
+            B_TRY(hipMemsetAsync(b->g_bar, 0, sizeof(unsigned) * gw, b->stream));
+            B_TRY(hipMemsetAsync(b->g_abort, 0, sizeof(int), b->stream));
         }
-        double *u0 = b->buf + (size_t)2 * b->n_total;  // fields 2..15 are contiguous
-        B_TRY(hipMemcpyAsync(b->snap, u0, sizeof(double) * nfield, hipMemcpyDeviceToDevice, b->stream));
-        B_TRY(hipMemcpyAsync(b->snap_pos, b->pos, sizeof(double2) * b->n_total, hipMemcpyDeviceToDevice,
-                             b->stream));
-        B_TRY(hipMemcpyAsync(b->snap_fail, b->fail, sizeof(uint32_t) * nwords, hipMemcpyDeviceToDevice, b->stream));
-        B_TRY(hipMemcpyAsync(b->snap_ret, b->d_retired, sizeof(int32_t) * nspec_sz, hipMemcpyDeviceToDevice,
-                             b->stream));
-    }
+        B_TRY(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
+        if (v.pause_below > 0) {
+            const int n_active = v.nqueue;
+            B_TRY(hipMemcpyAsync(b->d_active, &n_active, sizeof(int), hipMemcpyHostToDevice, b->stream));
+            B_TRY(hipMemsetAsync(b->d_paused, 0, sizeof(int32_t) * nspec, b->stream));
+        }
+        B_TRY(launch_batch_adr(b->stream, gw * w, block, b->group, v, du, tol, max_iters, b->queue, b->d_iters,
+                               b->d_resid, gr));
+        B_TRY(hipStreamSynchronize(b->stream));  // n_active lives on this frame
+        if (w > 1) {
+            int aborted = 0;
+            B_TRY(copy_on(b->stream, &aborted, b->g_abort, sizeof(int), hipMemcpyDeviceToHost));
+            if (aborted) {
+                int rc = batch_snapshot(b, false);
+                if (rc) return rc;
+                b->err = "a workgroup group barrier timed out (the group's workgroups were not all resident); the "
+                         "increment was abandoned and every specimen restored to its state before it";
+                return PD_E_HIP;
+            }
+        }
+        return PD_OK;
+    };
+    // Tail launches: once every specimen has been handed out and at most
+    // half of the device's workgroups still run (pause_below specimens of w
+    // workgroups), the running specimens pause at their next iteration and
+    // continue in a new launch with as many workgroups each as the device
+    // holds for them (at least twice as many, up to wtail), until w reaches
+    // wtail.
+    const auto pause_limit = [&](int w) { return tail && w < wtail ? std::max(1, capacity / (2 * w)) : 0; };
     HipEvents<2> ev;
     B_TRY(ev.create());
     hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
-    B_TRY(hipMemsetAsync(b->queue, 0, sizeof(unsigned), b->stream));
     B_TRY(hipEventRecord(e0, b->stream));
-    B_TRY(launch_batch_adr(b->stream, groups * W, block, b->group, b->view(), du, tol, max_iters, b->queue,
-                           b->d_iters, b->d_resid, gr));
+    BatchView v = b->view();
+    v.pause_below = pause_limit(W);
+    int w = W, gw = groups, tail_parts = 0;
+    std::vector<int32_t> flags((size_t)nspec);
+    for (;;) {
+        int rc = launch(v, w, gw);
+        if (rc) return rc;
+        if (v.pause_below == 0) break;
+        B_TRY(copy_on(b->stream, flags.data(), b->d_paused, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
+        std::vector<int32_t> paused;
+        for (int q = 0; q < nspec; ++q)
+            if (flags[q]) paused.push_back(q);
+        if (paused.empty()) break;
+        std::stable_sort(paused.begin(), paused.end(),
+                         [&](int32_t a, int32_t c) { return b->spec[a].nb > b->spec[c].nb; });
+        int wn = w;
+        while (2 * wn <= wtail && (int64_t)paused.size() * 2 * wn <= capacity) wn *= 2;
+        B_TRY(copy_on(b->stream, b->d_order2, paused.data(), sizeof(int32_t) * paused.size(),
+                      hipMemcpyHostToDevice));
+        v.order = b->d_order2;
+        v.nqueue = (int)paused.size();
+        v.resume = 1;
+        v.it0 = b->d_iters;
+        v.pause_below = wn > w ? pause_limit(wn) : 0;  // a launch that cannot widen is the last
+        w = wn;
+        gw = std::min<int>((int)paused.size(), capacity / w);
+        tail_parts = w;
+    }
     B_TRY(hipEventRecord(e1, b->stream));
     B_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
     B_TRY(hipEventElapsedTime(&ms, e0, e1));
-    if (W > 1) {
-        int aborted = 0;
-        B_TRY(copy_on(b->stream, &aborted, b->g_abort, sizeof(int), hipMemcpyDeviceToHost));
-        if (aborted) {
-            double *u0 = b->buf + (size_t)2 * b->n_total;
-            B_TRY(hipMemcpyAsync(u0, b->snap, sizeof(double) * nfield, hipMemcpyDeviceToDevice, b->stream));
-            B_TRY(hipMemcpyAsync(b->pos, b->snap_pos, sizeof(double2) * b->n_total, hipMemcpyDeviceToDevice,
-                                 b->stream));
-            B_TRY(hipMemcpyAsync(b->fail, b->snap_fail, sizeof(uint32_t) * nwords, hipMemcpyDeviceToDevice,
-                                 b->stream));
-            B_TRY(hipMemcpyAsync(b->d_retired, b->snap_ret, sizeof(int32_t) * nspec_sz, hipMemcpyDeviceToDevice,
-                                 b->stream));
-            B_TRY(hipStreamSynchronize(b->stream));
-            b->err = "a workgroup group barrier timed out (the group's workgroups were not all resident); the "
-                     "increment was abandoned and every specimen restored to its state before it";
-            return PD_E_HIP;
-        }
-    }
+    std::vector<int32_t> its((size_t)nspec);
+    B_TRY(copy_on(b->stream, its.data(), b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
     b->last_block = block;
     b->last_parts = W;
+    b->last_tail_parts = tail_parts;
     if (ms_out) *ms_out = ms;
-    if (iters_out) B_TRY(copy_on(b->stream, iters_out, b->d_iters, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
+    if (iters_out) std::copy(its.begin(), its.end(), iters_out);
     if (resid_out) B_TRY(copy_on(b->stream, resid_out, b->d_resid, sizeof(double) * nspec, hipMemcpyDeviceToHost));
+    return PD_OK;
+}
+
+int pd_batch_set_tail(pd_batch *b, int on)
+{
+    // 1 (default): an increment's slowest specimens finish in further
+    // launches with more workgroups each once the device is mostly idle;
+    // 0: one persistent launch per increment.  Results are identical.
+    if (!b || (on != 0 && on != 1)) return PD_E_INVALID;
+    b->tail = on;
+    return PD_OK;
+}
+
+int pd_batch_set_separation(pd_batch *b, double ratio)
+{
+    // 0 (default): off.  0 < ratio < 1: a specimen whose strip reaction falls
+    // below ratio x its reaction at its last converged increment has
+    // fractured through; its increment ends there and, with retire on, it is
+    // retired with flag 2.
+    if (!b || !(ratio >= 0.0 && ratio < 1.0)) return PD_E_INVALID;
+    b->sep = ratio;
+    return PD_OK;
+}
+
+int pd_batch_get_retired(pd_batch *b, int32_t *flags)
+{
+    // per specimen: 0 active, 1 retired unconverged, 2 retired separated
+    if (!b || !flags) return PD_E_INVALID;
+    B_TRY(copy_on(b->stream, flags, b->d_retired, sizeof(int32_t) * b->spec.size(), hipMemcpyDeviceToHost));
+    return PD_OK;
+}
+
+int pd_batch_get_tail_parts(pd_batch *b, int *parts)
+{
+    // workgroups per specimen of the last increment's second launch (0: none)
+    if (!b || !parts) return PD_E_INVALID;
+    *parts = b->last_tail_parts;
     return PD_OK;
 }
 
@@ -386,7 +524,7 @@ int pd_batch_set_parts(pd_batch *b, int parts)
     // specimen's share of the device (its work over W workgroups) is no
     // longer than the whole batch's.  Results are identical for every choice.
     if (!b) return PD_E_INVALID;
-    if (parts < 0 || parts > kMaxParts || (parts & (parts - 1))) {
+    if (parts < 0 || parts > kMaxFirstParts || (parts & (parts - 1))) {
         b->err = "parts must be 0, 1, 2, 4, 8 or 16";
         return PD_E_INVALID;
     }

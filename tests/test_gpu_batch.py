@@ -170,3 +170,60 @@ def test_batch_warm_start_through_fracture():
     assert broke[0][-1] > 2 * broke[0][0] and broke[1][-1] > 2 * broke[1][0]
     assert tot[1].sum() < 0.8 * tot[0].sum()
     assert abs(broke[1][-1] - broke[0][-1]) < 0.15 * broke[0][-1]
+
+
+@pytest.mark.parametrize("warm,du,parts", [(0, 1.0e-6, 1), (1, 2.0e-6, 2), (1, 2.0e-6, 4)])
+def test_batch_tail_launch_matches_one_launch(warm, du, parts):
+    # with few workgroups per specimen the device is mostly idle once the
+    # queue is empty: the running specimens pause and continue in launches
+    # with more workgroups each, resuming at the iteration they paused at,
+    # so every result equals the one-launch run (2 um increments break bonds
+    # from the sixth increment on)
+    p = PB.plate_params()
+    specs = PB.plate_family(12, n_min=24, n_max=100)
+    out = {}
+    for tail in (0, 1):
+        with PB.Batch(p, specs) as b:
+            b.set_warm(warm)
+            b.set_parts(parts)
+            b.set_tail(tail)
+            rows = []
+            for inc in range(8):
+                its, res, _ = b.increment(du, 1e-6, 40000, block=1024)
+                rows.append((its.copy(), res.copy(), b.tail_parts()))
+            out[tail] = (rows, [b.get(s) for s in range(len(specs))], [b.csr(s)[2] for s in range(len(specs))])
+    (ra, ua, fa), (rb, ub, fb) = out[0], out[1]
+    print("tail launch workgroups per specimen", [r[2] for r in rb])
+    assert all(r[2] == 0 for r in ra) and all(r[2] >= 16 for r in rb)
+    for (ia, sa, _), (ib, sb, _) in zip(ra, rb):
+        assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
+    for s in range(len(specs)):
+        assert np.array_equal(ua[s], ub[s]) and np.array_equal(fa[s], fb[s]), s
+
+
+def test_batch_separation_retires_fractured_plates():
+    # 5 um increments fracture most plates through within a few increments;
+    # with separation detection those end their increment once the strip
+    # reaction collapses (flag 2) instead of running to the iteration cap,
+    # and the specimens that never separate are unchanged
+    p = PB.plate_params()
+    specs = PB.plate_family(12, n_min=24, n_max=48)
+    runs = {}
+    for ratio in (0.0, 0.05):
+        with PB.Batch(p, specs) as b:
+            b.set_warm(1)
+            b.set_retire(1)
+            b.set_separation(ratio)
+            its_all = []
+            for inc in range(8):
+                its, res, _ = b.increment(5.0e-6, 1e-6, 20000, block=1024)
+                its_all.append(its.copy())
+            runs[ratio] = (np.array(its_all), b.retired(), [b.get(s) for s in range(len(specs))])
+    its0, ret0, u0 = runs[0.0]
+    its1, ret1, u1 = runs[0.05]
+    print("retired without separation", ret0.tolist(), "with", ret1.tolist())
+    print("iterations", its0.sum(), its1.sum())
+    assert (ret0 == 2).sum() == 0 and (ret1 == 2).sum() >= 3
+    assert its1.sum() < its0.sum()
+    for s in np.nonzero(ret1 == 0)[0]:
+        assert np.array_equal(its0[:, s], its1[:, s]) and np.array_equal(u0[s], u1[s])
