@@ -224,7 +224,7 @@ def batch_cpu_baseline(specs, du, tol, max_iters, T):
 
 
 def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log=False, max_iters=12000,
-                with_cpu=True):
+                with_cpu=True, separation=0.05):
     """Batched ADR timing and the warm-start comparison (see module doc)."""
     import numpy as np
     from pd_gpu1_amd import batch as PB
@@ -242,6 +242,8 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
             b.set_group(group)
             b.set_warm(mode)
             b.set_retire(1)
+            b.set_tail(32)
+            b.set_separation(separation)
             n_part, n_bonds, _ = b.info()
             incs = []
             for k in range(batch_increments):
@@ -253,7 +255,9 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                 info = [b.spec_info(s_) for s_ in range(len(specs))] if k == 0 else None
                 incs.append({"wall_ms": round(ms, 1), "active": int(active.sum()),
                              "iterations_median": int(np.median(its[active])) if active.any() else 0,
-                             "iterations_max": int(its.max()), "unconverged": unconv})
+                             "iterations_max": int(its.max()), "unconverged": unconv,
+                             "separated_so_far": int((b.retired() == 2).sum()),
+                             "tail_launch_workgroups_per_specimen": b.tail_parts()})
                 if info is not None:
                     evals = [int(i) + 1 for i in its]  # force evaluations: iterations + the converged check
                     incs[-1]["bond_iterations"] = int(sum(q[2] * e for q, e in zip(info, evals)))
@@ -273,13 +277,16 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                       "bytes_per_specimen_iteration": "20.125 Nb + 196 N (FP64 PMB force 20.125 Nb + 52 N, "
                                                       "ADR damping terms and update 144 N)",
-                      "workload": "increment 0 of the plain run (one launch), every specimen's force "
-                                  "evaluations = iterations + 1"}
+                      "workload": "increment 0 of the plain run (its persistent launches, tail launches "
+                                  "included), every specimen's force evaluations = iterations + 1"}
     cpu = batch_cpu_baseline(specs[:16], W.DU, 1e-6, max_iters, block) if with_cpu else None
     out["batched"] = {"workload": f"{n_batch} PMB plates, sides 40..200 (seed 0x5EED0003+k), {batch_increments} "
-                                  "increments of du=1e-6 m, ADR to relative residual 1e-6, one persistent launch "
-                                  f"per increment; a specimen that has not converged after {max_iters} iterations "
-                                  "(fractured through) is retired",
+                                  "increments of du=1e-6 m, ADR to relative residual 1e-6, persistent launches "
+                                  "(the slowest specimens of an increment continue in tail launches with more "
+                                  f"workgroups each); a specimen that has not converged after {max_iters} "
+                                  "iterations is retired, and one whose strip reaction falls below "
+                                  f"{separation:g} of its last converged reaction (fractured through) ends its "
+                                  "increment and is retired",
                       "particles": n_part, "bonds": n_bonds, "plain_s": tot["plain"] / 1e3,
                       "warm_s": tot["warm"] / 1e3, "speedup": tot["plain"] / tot["warm"],
                       "increments_plain": runs["plain"], "increments_warm": runs["warm"],

@@ -277,18 +277,19 @@ int pd_batch_set_warm(pd_batch *b, int mode);
  * separated, or needs more than max_iters) is retired; later increments skip
  * it and report 0 iterations (no reference counterpart). */
 int pd_batch_set_retire(pd_batch *b, int on);
-/* Tail launches (no reference counterpart): on = 1 (default).  The slowest
- * specimens of an increment set its wall time and would run on a few
- * workgroups while the rest of the device idles: once every specimen has
- * been handed out and at most half of the device's workgroups still run,
- * the running specimens pause at their next iteration and continue in a new
- * launch with at least twice the workgroups each (up to 64, and at least
- * 16 lanes of the damping tree per workgroup), repeatedly.  A
+/* Tail launches (no reference counterpart).  The slowest specimens of an
+ * increment set its wall time and would run on a few workgroups while the
+ * rest of the device idles: once every specimen has been handed out and at
+ * most half of the device's workgroups still run, the running specimens
+ * pause at their next iteration and continue in a new launch with at least
+ * twice the workgroups each, up to max_parts = 16, 32 (default) or 64 (and
+ * at least 16 lanes of the damping tree per workgroup), repeatedly.  A
  * paused specimen resumes at the iteration it paused at, so results are
- * identical to one launch.  0 = one persistent launch per increment; off
- * when pd_batch_increment is given a grid.  pd_batch_get_tail_parts: workgroups per specimen of the last
- * increment's last tail launch (0: there was none). */
-int pd_batch_set_tail(pd_batch *b, int on);
+ * identical to one launch.  max_parts = 0: one persistent launch per
+ * increment; off when pd_batch_increment is given a grid.
+ * pd_batch_get_tail_parts: workgroups per specimen of the last increment's
+ * last tail launch (0: there was none). */
+int pd_batch_set_tail(pd_batch *b, int max_parts);
 int pd_batch_get_tail_parts(pd_batch *b, int *parts);
 /* Separation (no reference counterpart; the reference runs a fixed number of
  * ADR iterations per load step, Solve_Plate_Tensile.cu:106): 0 < ratio < 1

@@ -129,10 +129,11 @@ class Batch:
         """1: start each increment from u + the previous increment (0: plain)."""
         self._c(lib().pd_batch_set_warm(self._h, mode))
 
-    def set_tail(self, on):
-        """1: finish an increment's slowest specimens in further launches with
-        more workgroups each once the device is mostly idle (0: one launch)."""
-        self._c(lib().pd_batch_set_tail(self._h, int(on)))
+    def set_tail(self, max_parts):
+        """16, 32 or 64: finish an increment's slowest specimens in further
+        launches with up to max_parts workgroups each once the device is half
+        idle (0: one launch)."""
+        self._c(lib().pd_batch_set_tail(self._h, int(max_parts)))
 
     def tail_parts(self):
         w = C.c_int()
@@ -205,8 +206,8 @@ def main(argv=None):
     ap.add_argument("--parts", type=int, default=0, help="workgroups per specimen (0: sized from the work)")
     ap.add_argument("--warm", type=int, default=0)
     ap.add_argument("--retire", type=int, default=1)
-    ap.add_argument("--tail", type=int, default=1,
-                    help="1: finish the slowest specimens in further launches with more workgroups (0: one launch)")
+    ap.add_argument("--tail", type=int, default=32,
+                    help="widest tail launch, workgroups per specimen: 16, 32 or 64 (0: one launch)")
     ap.add_argument("--separation", type=float, default=0.0,
                     help="end a specimen's increment once its strip reaction falls below this fraction of its last "
                          "converged reaction (0: off)")

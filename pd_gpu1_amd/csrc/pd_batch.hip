@@ -33,7 +33,7 @@ struct pd_batch {
     int32_t *d_retired = nullptr; // per specimen: 1 once an increment ended unconverged, 2 separated (retire on)
     double *d_react = nullptr;    // per specimen: strip reaction at its last converged increment
     int32_t *d_order2 = nullptr;  // queue of the paused specimens a tail launch continues
-    int tail = 1;                 // 1: tail launches (pd_batch_set_tail)
+    int tail = 32;                // widest tail launch in workgroups per specimen (0: none)
     double sep = 0.0;             // separation ratio (0: off)
     int *d_active = nullptr;      // specimens of a launch not yet ended or paused
     int32_t *d_paused = nullptr;  // [nspec] specimens a launch paused
@@ -337,7 +337,7 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     const int capacity = dev_cu * batch_resident_per_cu(block, b->group);
     int W = b->parts;
     const int wmax = std::min(kMaxFirstParts, block / 64);
-    const int wtail = std::min(kMaxParts, block / kMinPartLanes);
+    const int wtail = std::min(std::min(kMaxParts, b->tail), block / kMinPartLanes);
     if (W == 0) {
         std::vector<int32_t> ret((size_t)nspec, 0);
         if (b->retire)
@@ -460,13 +460,14 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     return PD_OK;
 }
 
-int pd_batch_set_tail(pd_batch *b, int on)
+int pd_batch_set_tail(pd_batch *b, int max_parts)
 {
-    // 1 (default): an increment's slowest specimens finish in further
-    // launches with more workgroups each once the device is mostly idle;
-    // 0: one persistent launch per increment.  Results are identical.
-    if (!b || (on != 0 && on != 1)) return PD_E_INVALID;
-    b->tail = on;
+    // 16, 32 (default) or 64: an increment's slowest specimens finish in
+    // further launches with up to max_parts workgroups each once the device
+    // is half idle; 0: one persistent launch per increment.  Results are
+    // identical.
+    if (!b || (max_parts != 0 && max_parts != 16 && max_parts != 32 && max_parts != 64)) return PD_E_INVALID;
+    b->tail = max_parts;
     return PD_OK;
 }
 
