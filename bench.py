@@ -182,6 +182,7 @@ def cpu_full_configs(one_thread_steps=(50, 5, 1)):
         for _ in range(sc.steps):
             m.step()
         run = time.perf_counter() - t0
+        broken = int((m.fail == 0).sum())  # after the configuration's steps
         threads = O.get_threads()
         O.set_threads(1)
         t0 = time.perf_counter()
@@ -193,11 +194,34 @@ def cpu_full_configs(one_thread_steps=(50, 5, 1)):
         rows[name] = {"particles": len(sc.x), "bonds": nb, "steps": sc.steps, "setup_s": setup, "run_s": run,
                       "bond_updates_per_s": nb * sc.steps / run, "threads": threads,
                       "one_thread": {"steps": n1, "s": run1, "bond_updates_per_s": nb * n1 / run1},
-                      "broken_bonds_at_end": int((m.fail == 0).sum())}
+                      "broken_bonds_at_end": broken}
         print(f"[bench] oracle {name}: {sc.steps} steps in {run:.1f} s on {threads} threads", file=sys.stderr,
               flush=True)
         del m
     return {"kind": "port", **rows, **cpu_model()}
+
+
+def warm_cpu_baseline(spec, gpu_s, gpu_its, T=4096):
+    """FP64 oracle plain ADR on the first increment of one evaluation plate of
+    the warm-start comparison (the single-problem path's 4096-lane reduction
+    tree), beside the GPU's plain ADR on the same increment."""
+    from oracle import oracle as O
+    from oracle.runner import adr_increment, plate_specimen
+    from pd_gpu1_amd import batch as PB
+    from pd_gpu1_amd import scenarios
+    from pd_gpu1_amd import warmstart as W
+
+    n, _, clen, ang = spec
+    m = plate_specimen(scenarios.params_dict(PB.plate_params(W.DX)), n, clen, ang)
+    t0 = time.perf_counter()
+    its, _ = adr_increment(m, W.DU, 1e-6, 400000, T)
+    t = time.perf_counter() - t0
+    work = len(m.col) * (its + 1)
+    return {"value": work / t, "unit": "bond-iterations/s", "cores": O.get_threads(), "kind": "port",
+            "sample": f"FP64 oracle plain ADR, first increment of the first evaluation plate ({n} x {n} free "
+                      f"particles), {its} iterations, {t:.1f} s",
+            "oracle_iterations": its, "gpu_iterations": gpu_its, "gpu_plain_s": gpu_s,
+            "gpu_bond_iterations_per_s": work / gpu_s if its == gpu_its else None}
 
 
 def batch_cpu_baseline(specs, du, tol, max_iters, T):
@@ -315,6 +339,8 @@ def adr_section(n_batch=64, batch_increments=2, n_eval=2, eval_increments=6, log
         "network_weights": "pd_gpu1_amd/data/mpnn_warmstart.npy" if have_net else None,
         "network_scale": W.NET_SCALE,
     }
+    if with_cpu:
+        out["warm_start"]["cpu_baseline"] = warm_cpu_baseline(specs[0], float(rows[0, 5]), int(rows[0, 2]))
     return out
 
 
