@@ -205,6 +205,115 @@ def samples(rec, delta):
         yield x, ef, t, free
 
 
+class SideGraph:
+    """What every plate of side n shares: the lattice with its strips, the
+    bond graph (CSR), the normalized positions, the uniform-stretch
+    increment and the static edge features (reference bond vector / delta;
+    column 2, the intact flag, is filled per sample)."""
+
+    def __init__(self, n, rp, col, delta):
+        self.n = n
+        self.x, self.y, self.n_int, self.top, self.bot = plate_layout(n)
+        self.rp, self.col = rp, col
+        self.pos = normalized_positions(self.x, self.y, self.n_int)
+        self.half = plate_half(self.x, self.y, self.n_int)
+        self.base = base_increment(self.x, self.y, self.n_int, self.top)
+        self.ef = edge_features(self.x, self.y, rp, col, np.ones(len(col), np.uint8), delta)
+        self.free = np.zeros(len(self.x), np.float32)
+        self.free[: self.n_int] = 1.0
+
+
+def generate_first(specs, sides, delta, tol=1e-6, max_iters=400000, block=1024, group=4, log=None):
+    """First increments of ``specs`` by the batched ADR, kept compactly: per
+    specimen its side, crack, intact mask before the increment (packed bits)
+    and the target (u_1 - du_base) / du as float32 [N][2]; the graph of each
+    side goes into ``sides`` (n -> SideGraph) once.  Returns (records,
+    iterations, seconds)."""
+    p = PB.plate_params(DX)
+    recs = []
+    t0 = time.perf_counter()
+    with PB.Batch(p, specs) as b:
+        b.set_group(group)
+        masks = []
+        for s, (n, _, clen, ang) in enumerate(specs):
+            rp, col, mask = b.csr(s)
+            if n not in sides:
+                sides[n] = SideGraph(n, rp, col, delta)
+            masks.append(np.packbits(mask, bitorder="little"))
+        its, res, ms = b.increment(DU, tol, max_iters, block=block)
+        for s, (n, _, clen, ang) in enumerate(specs):
+            u1 = b.get(s, "U")
+            t = ((u1 - sides[n].base) / DU).T.astype(np.float32)
+            recs.append(dict(n=n, clen=clen, angle=ang, mask=masks[s], nb=len(sides[n].col), t=t))
+        if log:
+            log(f"[data] {len(specs)} first increments: iterations median {int(np.median(its))} max {its.max()}, "
+                f"{ms:.0f} ms, max residual {res.max():.2e}")
+    return recs, its, time.perf_counter() - t0
+
+
+def first_sample(rec, side):
+    """(node features, edge features, target, weight) of a compact
+    first-increment record; the edge features are the side's array with the
+    intact flags of this record written into column 2 (valid until the
+    side's next sample)."""
+    mask = np.unpackbits(rec["mask"], count=rec["nb"], bitorder="little")
+    cf = crack_features(side.pos, rec["clen"], rec["angle"], side.half)
+    x = node_features(side.pos, np.zeros((2, len(side.x))), damage(side.rp, mask), side.top, side.bot, cf)
+    side.ef[:, 2] = mask
+    return x, side.ef, rec["t"], side.free
+
+
+def epoch_order(recs, rng, window=512):
+    """A random permutation of the records, sorted by side within windows of
+    `window` consecutive records so that the graph changes less often."""
+    perm = rng.permutation(len(recs))
+    out = []
+    for k in range(0, len(perm), window):
+        w = perm[k:k + window]
+        out.extend(w[np.argsort([recs[i]["n"] for i in w], kind="stable")])
+    return out
+
+
+def train_first(net, recs, sides, epochs, lr=2e-3, lr_end=5e-5, seed=0, log=None, max_seconds=None, window=512):
+    """Adam over compact first-increment records (generate_first).  The
+    learning rate decays geometrically from lr to lr_end with the training's
+    progress, the larger of the fraction of steps done and, with
+    max_seconds, the fraction of that time used; training stops after the
+    last step or once max_seconds have passed.  Returns the mean loss of each
+    (possibly partial) epoch and the number of steps taken."""
+    rng = np.random.default_rng(seed)
+    total = epochs * len(recs)
+    step = 0
+    hist = []
+    t0 = time.perf_counter()
+    cur = None
+    for ep in range(epochs):
+        losses = []
+        for i in epoch_order(recs, rng, window):
+            used = (time.perf_counter() - t0) / max_seconds if max_seconds else 0.0
+            if used >= 1.0:
+                break
+            prog = max(step / max(total - 1, 1), used)
+            r = recs[i]
+            side = sides[r["n"]]
+            if cur is not side:
+                net.set_graph(side.rp, side.col)
+                cur = side
+            x, ef, t, w = first_sample(r, side)
+            net.set_features(x, ef, t, w)
+            losses.append(net.train_step(lr * (lr_end / lr) ** prog))
+            step += 1
+            if log and step % 2000 == 0:
+                log(f"[train] step {step}: loss (last 2000) {np.mean(losses[-2000:]):.4e}, "
+                    f"{time.perf_counter() - t0:.0f} s")
+        hist.append(float(np.mean(losses)) if losses else float("nan"))
+        if log:
+            log(f"[train] epoch {ep}: mean loss {hist[-1]:.4e} over {len(losses)} samples")
+        if len(losses) < len(recs):
+            break
+    return hist, step
+
+
 def train(net, recs, delta, epochs, lr=1e-3, lr_end=1e-4, seed=0, log=None):
     """Adam over all (specimen, increment) samples; the learning rate decays
     geometrically from lr to lr_end.  Returns the mean loss of each epoch."""

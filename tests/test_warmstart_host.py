@@ -166,3 +166,38 @@ def test_race_keeps_the_guess_when_it_wins_or_both_fail():
     its, r, b_won = W.race(a, "ga", b, "gb", 1e-6, 500, chunk=128)
     assert not b_won and its == 500
     assert [c[1] for c in a.prob.calls] == [128, 128, 128, 116]
+
+
+def test_compact_first_increment_sample_equals_the_full_record():
+    n = 4
+    x, y, n_int, top, bot = W.plate_layout(n)
+    N = len(x)
+    P = np.stack([x, y], 1)
+    d2 = ((P[:, None] - P[None]) ** 2).sum(-1)
+    rows = [np.nonzero((d2[i] < (3.015 * W.DX) ** 2) & (np.arange(N) != i))[0] for i in range(N)]
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    rng = np.random.default_rng(3)
+    mask = (rng.uniform(size=len(col)) > 0.2).astype(np.uint8)
+    u1 = W.DU * rng.uniform(size=(2, N))
+    delta = 3.015 * W.DX
+    rec = dict(n=n, n_int=n_int, rp=rp, col=col, x=x, y=y, top=top, bot=bot, u=[np.zeros((2, N)), u1],
+               clen=1e-3, angle=0.2, mask=[mask, mask])
+    x_full, ef_full, t_full, w_full = next(W.samples(rec, delta))
+    side = W.SideGraph(n, rp, col, delta)
+    comp = dict(n=n, clen=1e-3, angle=0.2, mask=np.packbits(mask, bitorder="little"), nb=len(col),
+                t=((u1 - side.base) / W.DU).T.astype(np.float32))
+    xc, efc, tc, wc = W.first_sample(comp, side)
+    np.testing.assert_array_equal(xc, x_full)
+    np.testing.assert_array_equal(efc, ef_full)
+    np.testing.assert_array_equal(tc, t_full)
+    np.testing.assert_array_equal(wc, w_full)
+
+
+def test_epoch_order_is_a_permutation_sorted_by_side_within_windows():
+    recs = [dict(n=int(v)) for v in np.random.default_rng(5).integers(40, 200, size=1000)]
+    order = W.epoch_order(recs, np.random.default_rng(0), window=128)
+    assert sorted(order) == list(range(1000))
+    for k in range(0, 1000, 128):
+        sides = [recs[i]["n"] for i in order[k:k + 128]]
+        assert sides == sorted(sides)
