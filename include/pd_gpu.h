@@ -273,6 +273,16 @@ int pd_batch_choose_parts(const int64_t *nb, const int32_t *n, const int32_t *re
  * free particles also move by the previous increment, which is exact while
  * the response is linear; ADR still converges every specimen to tol. */
 int pd_batch_set_warm(pd_batch *b, int mode);
+/* Bond coefficients of the batched force phase (no reference counterpart):
+ * -1 (default) = recompute each bond's {kappa, |xi|} from the reference
+ * positions with the set-up's own expressions (Surf_Corr.cu:312-337 and
+ * kappa = vol*fac*bc*scr) when pd_batch_create found that recomputation
+ * equal to the stored values bit for bit (no HBM stream of coefficients),
+ * 0 = always stream them, 1 = recompute (PD_E_INVALID when they differ).
+ * Results are identical either way.  pd_batch_coef_info: *recomputed = 1
+ * when the next increment recomputes them. */
+int pd_batch_set_coef_mode(pd_batch *b, int mode);
+int pd_batch_coef_info(pd_batch *b, int *recomputed);
 /* on = 1: a specimen whose increment ends without reaching tol (it has
  * separated, or needs more than max_iters) is retired; later increments skip
  * it and report 0 iterations (no reference counterpart). */

@@ -151,6 +151,17 @@ class Batch:
         self._c(lib().pd_batch_get_retired(self._h, out.ctypes.data_as(_lib.I32)))
         return out
 
+    def set_coef_mode(self, mode):
+        """-1: recompute the bond coefficients from the reference positions
+        when they equal the stored ones bit for bit (default), 0: stream them,
+        1: recompute.  Results are identical."""
+        self._c(lib().pd_batch_set_coef_mode(self._h, mode))
+
+    def coef_recomputed(self):
+        r = C.c_int()
+        self._c(lib().pd_batch_coef_info(self._h, C.byref(r)))
+        return bool(r.value)
+
     def set_retire(self, on):
         """1: skip specimens from the increment after one that ended unconverged."""
         self._c(lib().pd_batch_set_retire(self._h, on))

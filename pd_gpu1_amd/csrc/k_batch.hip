@@ -53,16 +53,42 @@ __device__ __forceinline__ void block_canon4(double p[4], double (*sh)[BT / 64],
     }
 }
 
+// Coefficients {kappa, |xi|} of bond k with neighbour c of a row at x0, x1:
+// streamed from KD, or (RC) recomputed from the reference positions XY with
+// the set-up's own FP64 expressions -- |xi| and the partial-volume factor of
+// k_vol_corr (Surf_Corr.cu:312-337), kappa = vol*fac*bc*scr of k_pmb_coef
+// with scr = 1 (no surface correction in the plate family) -- which give
+// the stored values bit for bit; the batch checks that once per set-up.
+// Recomputing takes the 16-byte coefficient stream of every bond off HBM for
+// a 16-byte gather of a read-only position that the group's L2 holds.
+template <bool RC>
+__device__ __forceinline__ double2 bond_coef(int64_t k, int c, const double2 *KD, const double2 *XY, double x0,
+                                             double x1, const BatchCoef &cg)
+{
+    if (!RC) return KD[k];
+    const double2 xc = XY[c];
+    const double ddx = xc.x - x0, ddy = xc.y - x1;
+    const double id = sqrt(ddx * ddx + ddy * ddy);
+    const double lo = cg.delta - cg.dx / 2.0, hi = cg.delta + cg.dx / 2.0;
+    double fac;
+    if (id <= lo)
+        fac = 1.0;
+    else if (id <= hi)
+        fac = (hi - id) / cg.dx;
+    else
+        fac = 0.0;
+    return make_double2(cg.vol * fac * cg.bc, id);  // times scr = 1, which changes no bit
+}
+
 // Pair force term of bond k of a row at (x0+u0, x1+u1), Force_Bond.cu:185-196
-// operand order; breaks the bond when its stretch exceeds s0.  P[c] holds
-// (X[c] + UX[c], Y[c] + UY[c]), the first sum of that order, so the
-// neighbour costs one 16-byte load.
-__device__ __forceinline__ void bond_term(int64_t k, int c, const double2 *KD, double x0, double x1, double u0,
+// operand order, kd = {kappa, |xi|} of the bond; breaks the bond when its
+// stretch exceeds s0.  P[c] holds (X[c] + UX[c], Y[c] + UY[c]), the first sum
+// of that order, so the neighbour costs one 16-byte load.
+__device__ __forceinline__ void bond_term(int64_t k, int c, double2 kd, double x0, double x1, double u0,
                                           double u1, const double2 *P, double s0, uint32_t *fail, double &tx,
                                           double &ty)
 {
     const double2 pc = P[c];
-    const double2 kd = KD[k];
     double d0 = pc.x - x0 - u0;
     double d1 = pc.y - x1 - u1;
     double nl = sqrt(d0 * d0 + d1 * d1);
@@ -115,7 +141,7 @@ __device__ __forceinline__ bool group_sync(const BatchGroups &gr, int grp, unsig
     return ok_sh != 0;
 }
 
-template <int BT, int G>
+template <int BT, int G, bool RC>
 __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double tol, int max_iters,
                                                   unsigned *queue, int32_t *iters_out, double *resid_out,
                                                   BatchGroups gr)
@@ -127,7 +153,11 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
     __shared__ double q_sh[kMaxParts][4];
     const int tid = threadIdx.x;
     const int W = gr.parts;                       // workgroups per specimen
-    const int grp = (int)blockIdx.x / W, part = (int)blockIdx.x % W;
+    // Workgroups reach the 8 XCDs round-robin; with gr.xcd the renumbering
+    // b -> (b mod 8)(grid/8) + b/8 gives the W workgroups of a group one XCD,
+    // so a specimen's particle state sits in one L2.
+    const int b = gr.xcd ? (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const int grp = b / W, part = b % W;
     const int gt = part * BT + tid, GT = W * BT;  // thread within the group, threads of the group
     const int L = BT / W;                         // damping-tree lanes of this part
     unsigned arrivals = 0;
@@ -173,6 +203,7 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
         const int32_t *rp = B.row_ptr + d.rp_off;
         const int32_t *col = B.col + d.b_off;
         const double2 *KD = B.kd + d.b_off;
+        const double2 *XY = B.xy + d.p_off;
         uint32_t *fail = B.fail + (d.b_off >> 5);
         const double mass = B.mass, dt = B.dt, s0 = B.s0;
         double *DUPX = B.dupx + d.p_off, *DUPY = B.dupy + d.p_off;
@@ -221,7 +252,9 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                     for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
                         if (!intact(fail, k)) continue;
                         double tx, ty;
-                        bond_term(k, col[k], KD, x0, x1, u0, u1, P, s0, fail, tx, ty);
+                        const int c = col[k];
+                        bond_term(k, c, bond_coef<RC>(k, c, KD, XY, x0, x1, B.coef), x0, x1, u0, u1, P, s0, fail, tx,
+                                  ty);
                         fx += tx;
                         fy += ty;
                     }
@@ -247,7 +280,9 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
                         const int cn = kn < ke ? col[kn] : 0;
                         const int okn = kn < ke && intact(fail, kn);
                         double tx = 0.0, ty = 0.0;
-                        if (ok) bond_term(k, c, KD, x0, x1, u0, u1, P, s0, fail, tx, ty);
+                        if (ok)
+                            bond_term(k, c, bond_coef<RC>(k, c, KD, XY, x0, x1, B.coef), x0, x1, u0, u1, P, s0, fail,
+                                      tx, ty);
                         // add the terms in bond order, as the sequential loop does
                         const int cnt = ke - k0 < G ? ke - k0 : G;
 #pragma unroll
@@ -404,19 +439,48 @@ __global__ void k_pack_bond_pairs(int64_t nb, const double *__restrict__ kappa, 
     if (k < nb) kd[k] = make_double2(kappa[k], idist[k]);
 }
 
-template <int BT>
+// Per bond of one specimen, thread per row: 1 is added to *bad for every bond
+// whose stored {kappa, |xi|} differs from bond_coef's recomputation.
+__global__ void k_batch_coef_check(int n, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                   const double2 *__restrict__ kd, const double2 *__restrict__ xy, BatchCoef cg,
+                                   unsigned *bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned nbad = 0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const double2 r = bond_coef<true>(k, col[k], kd, xy, xy[i].x, xy[i].y, cg);
+        const double2 s = kd[k];
+        nbad += __double_as_longlong(r.x) != __double_as_longlong(s.x) ||
+                __double_as_longlong(r.y) != __double_as_longlong(s.y);
+    }
+    if (nbad) atomicAdd(bad, nbad);
+}
+
+template <int BT, bool RC>
 hipError_t batch_g(hipStream_t st, int grid, int group, const BatchView &B, double du, double tol, int max_iters,
                    unsigned *queue, int32_t *iters_out, double *resid_out, const BatchGroups &gr)
 {
+#define PD_BATCH_CASE(GV)                                                                                    \
+    case GV: k_batch_adr<BT, GV, RC><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
     switch (group) {
-    case 1: k_batch_adr<BT, 1><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
-    case 4: k_batch_adr<BT, 4><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
-    case 8: k_batch_adr<BT, 8><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
-    case 16: k_batch_adr<BT, 16><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
-    case 32: k_batch_adr<BT, 32><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
+    PD_BATCH_CASE(1) PD_BATCH_CASE(4) PD_BATCH_CASE(8) PD_BATCH_CASE(16) PD_BATCH_CASE(32)
     default: return hipErrorInvalidValue;
     }
+#undef PD_BATCH_CASE
     return hipGetLastError();
+}
+
+template <bool RC>
+hipError_t batch_b(hipStream_t st, int grid, int block, int group, const BatchView &B, double du, double tol,
+                   int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out, const BatchGroups &gr)
+{
+    switch (block) {
+    case 256: return batch_g<256, RC>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    case 512: return batch_g<512, RC>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    case 1024: return batch_g<1024, RC>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    default: return hipErrorInvalidValue;
+    }
 }
 } // namespace
 
@@ -428,14 +492,24 @@ hipError_t launch_pack_bond_pairs(hipStream_t st, int64_t nb, const double *kapp
     return hipGetLastError();
 }
 
-// Workgroups of k_batch_adr<block, group> that fit on one CU at once.
-int batch_resident_per_cu(int block, int group)
+hipError_t launch_batch_coef_check(hipStream_t st, int n, const int32_t *row_ptr, const int32_t *col,
+                                   const double2 *kd, const double2 *xy, const BatchCoef &cg, unsigned *bad)
+{
+    if (n <= 0) return hipSuccess;
+    k_batch_coef_check<<<(n + 255) / 256, 256, 0, st>>>(n, row_ptr, col, kd, xy, cg, bad);
+    return hipGetLastError();
+}
+
+// Workgroups of k_batch_adr<block, group, recompute> that fit on one CU at once.
+int batch_resident_per_cu(int block, int group, bool recompute)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
 #define PD_OCC(BT, GV)                                                                                      \
     if (block == BT && group == GV)                                                                     \
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(k_batch_adr<BT, GV>), BT, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                               \
+            &n, recompute ? reinterpret_cast<const void *>(k_batch_adr<BT, GV, true>)                   \
+                          : reinterpret_cast<const void *>(k_batch_adr<BT, GV, false>), BT, 0);
     PD_OCC(256, 1) PD_OCC(256, 4) PD_OCC(256, 8) PD_OCC(256, 16) PD_OCC(256, 32)
     PD_OCC(512, 1) PD_OCC(512, 4) PD_OCC(512, 8) PD_OCC(512, 16) PD_OCC(512, 32)
     PD_OCC(1024, 1) PD_OCC(1024, 4) PD_OCC(1024, 8) PD_OCC(1024, 16) PD_OCC(1024, 32)
@@ -450,12 +524,10 @@ hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, cons
     // W must divide the grid and the damping tree (block/64 waves)
     if (gr.parts < 1 || gr.parts > kMaxParts || (gr.parts & (gr.parts - 1)) || block / kMinPartLanes < gr.parts ||
         grid % gr.parts || (gr.parts > 1 && (!gr.bar || !gr.spec || !gr.sum || !gr.abort || !gr.flag)) ||
-        (B.pause_below > 0 && (!B.active || !B.paused)))
+        (gr.xcd && grid % (8 * gr.parts)) || (B.pause_below > 0 && (!B.active || !B.paused)) ||
+        (B.recompute && !B.xy))
         return hipErrorInvalidValue;
-    switch (block) {
-    case 256: return batch_g<256>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
-    case 512: return batch_g<512>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
-    case 1024: return batch_g<1024>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
-    default: return hipErrorInvalidValue;
-    }
+    return B.recompute
+               ? batch_b<true>(st, grid, block, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr)
+               : batch_b<false>(st, grid, block, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
 }

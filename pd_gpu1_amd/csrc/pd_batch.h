@@ -10,6 +10,11 @@ struct BatchSpec {
     int64_t nb;                  // bonds of this specimen
 };
 
+// Constants of the bond coefficients recomputed in the kernel (bond_coef).
+struct BatchCoef {
+    double delta, dx, vol, bc;
+};
+
 struct BatchView {
     int nspec;
     const BatchSpec *spec;
@@ -32,6 +37,9 @@ struct BatchView {
     double *react;        // [nspec] strip reaction |F| at the specimen's last converged increment (0: none)
     const int32_t *row_ptr, *col;
     const double2 *kd;    // per bond {kappa, |xi|}
+    const double2 *xy;    // per particle reference position (x, y)
+    int recompute;        // 1: kappa and |xi| recomputed from xy (bit-equal to kd, checked at set-up)
+    BatchCoef coef;
     uint32_t *fail;
     double mass, dt, s0;
 };
@@ -52,11 +60,14 @@ struct BatchGroups {
     int *flag;      // [groups] part 0's pause decision, read by every part after the group barrier
     int *abort;     // set when a group barrier times out
     uint64_t spin_ticks;  // longest wait at a group barrier, in wall-clock ticks
+    int xcd;        // 1: the grid is a multiple of 8 x parts and each group's workgroups share an XCD
 };
 
 hipError_t launch_pack_bond_pairs(hipStream_t st, int64_t nb, const double *kappa, const double *idist,
                                   double2 *kd);
-int batch_resident_per_cu(int block, int group);
+hipError_t launch_batch_coef_check(hipStream_t st, int n, const int32_t *row_ptr, const int32_t *col,
+                                   const double2 *kd, const double2 *xy, const BatchCoef &cg, unsigned *bad);
+int batch_resident_per_cu(int block, int group, bool recompute);
 hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
                             double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out,
                             const BatchGroups &gr);

@@ -26,6 +26,9 @@ struct pd_batch {
     double2 *pos = nullptr;       // current positions, refreshed by the kernel
     int32_t *row_ptr = nullptr, *col = nullptr;
     double2 *kd = nullptr;        // per bond {kappa, |xi|}
+    double2 *xy = nullptr;        // per particle reference position (x, y)
+    int coef_ok = 0;              // every stored {kappa, |xi|} equals its recomputation from xy
+    int coef_mode = -1;           // -1: recompute when coef_ok, 0: stream kd, 1: recompute
     uint32_t *fail = nullptr;
     unsigned *queue = nullptr;
     int32_t *d_iters = nullptr;
@@ -56,6 +59,7 @@ struct pd_batch {
     double *snap_react = nullptr;
     double barrier_timeout = 20.0;  // seconds a workgroup waits at a group barrier
     int warm = 0;                 // 1: increments start from u + the previous increment
+    int recompute() const { return coef_mode == 1 || (coef_mode == -1 && coef_ok); }
     BatchView view() const
     {
         BatchView v{};
@@ -80,6 +84,9 @@ struct pd_batch {
         v.sep = sep;
         v.react = d_react;
         v.row_ptr = row_ptr; v.col = col; v.kd = kd; v.fail = fail;
+        v.xy = xy;
+        v.recompute = recompute();
+        v.coef = BatchCoef{P.delta, P.dx, P.vol, P.bc};
         v.mass = P.mass; v.dt = P.dt; v.s0 = P.s0;
         return v;
     }
@@ -96,7 +103,7 @@ struct pd_batch {
 
 static void batch_free(pd_batch *b)
 {
-    void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->fail, b->queue, b->d_iters,
+    void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->xy, b->fail, b->queue, b->d_iters,
                     b->d_resid, b->d_retired, b->d_react, b->d_order2, b->snap_react, b->d_active, b->d_paused,
                     b->g_flag, b->g_bar, b->g_spec,
                     b->g_sum, b->g_abort, b->snap, b->snap_pos,
@@ -169,6 +176,7 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
     if (!rc) rc = alloc((void **)&b->row_ptr, sizeof(int32_t) * b->rp_total);
     if (!rc) rc = alloc((void **)&b->col, sizeof(int32_t) * b->nb_total);
     if (!rc) rc = alloc((void **)&b->kd, sizeof(double2) * b->nb_total);
+    if (!rc) rc = alloc((void **)&b->xy, sizeof(double2) * b->n_total);
     if (!rc) rc = alloc((void **)&b->fail, sizeof(uint32_t) * (b->nb_total / 32 + 1));
     if (!rc) rc = alloc((void **)&b->d_spec, sizeof(BatchSpec) * nspec);
     if (!rc) rc = alloc((void **)&b->d_order, sizeof(int32_t) * nspec);
@@ -217,6 +225,25 @@ int pd_batch_create(const pd_params *p, const pd_specimen *specs, int nspec, int
     }
     if (!rc && copy_on(b->stream, b->d_spec, b->spec.data(), sizeof(BatchSpec) * nspec, hipMemcpyHostToDevice) != hipSuccess)
         rc = PD_E_HIP;
+    if (!rc) {
+        // reference positions as (x, y) pairs, and whether the kernel may
+        // recompute every bond's {kappa, |xi|} from them bit for bit
+        hipError_t e = launch_pack_bond_pairs(b->stream, b->n_total, b->buf, b->buf + b->n_total, b->xy);
+        if (e == hipSuccess) e = fill_on(b->stream, b->queue, 0, sizeof(unsigned));
+        const BatchCoef cg{b->P.delta, b->P.dx, b->P.vol, b->P.bc};
+        for (int s = 0; s < nspec && e == hipSuccess; s++) {
+            const BatchSpec &d = b->spec[s];
+            e = launch_batch_coef_check(b->stream, d.n, b->row_ptr + d.rp_off, b->col + d.b_off, b->kd + d.b_off,
+                                        b->xy + d.p_off, cg, b->queue);
+        }
+        unsigned bad = 1;
+        if (e == hipSuccess) e = copy_on(b->stream, &bad, b->queue, sizeof(unsigned), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            b->err = std::string("batch coefficients: ") + hipGetErrorString(e);
+            rc = PD_E_HIP;
+        }
+        b->coef_ok = bad == 0;
+    }
     if (!rc) {
         // the largest plates set the wall time of an increment: hand them out first
         std::vector<int32_t> ord((size_t)nspec);
@@ -334,7 +361,7 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     // every workgroup resident at once (a persistent grid): the capacity
     int dev_cu = 0;
     B_TRY(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, b->device));
-    const int capacity = dev_cu * batch_resident_per_cu(block, b->group);
+    const int capacity = dev_cu * batch_resident_per_cu(block, b->group, b->recompute() != 0);
     int W = b->parts;
     const int wmax = std::min(kMaxFirstParts, block / 64);
     const int wtail = std::min(std::min(kMaxParts, b->tail), block / kMinPartLanes);
@@ -381,6 +408,7 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
             int rc = ensure_groups(b, gw);
             if (rc) return rc;
             gr = BatchGroups{w, b->g_bar, b->g_spec, b->g_sum, b->g_flag, b->g_abort, spin};
+            gr.xcd = (gw * w) % (8 * w) == 0;  // each group's workgroups on one XCD
             B_TRY(hipMemsetAsync(b->g_bar, 0, sizeof(unsigned) * gw, b->stream));
             B_TRY(hipMemsetAsync(b->g_abort, 0, sizeof(int), b->stream));
         }
@@ -495,6 +523,28 @@ int pd_batch_get_tail_parts(pd_batch *b, int *parts)
     // workgroups per specimen of the last increment's second launch (0: none)
     if (!b || !parts) return PD_E_INVALID;
     *parts = b->last_tail_parts;
+    return PD_OK;
+}
+
+int pd_batch_set_coef_mode(pd_batch *b, int mode)
+{
+    // -1 (default): the force phase recomputes every bond's {kappa, |xi|}
+    // from the reference positions when set-up found the recomputation equal
+    // to the stored values bit for bit; 0: always streamed; 1: recomputed
+    // (PD_E_INVALID when they differ).  Results are identical either way.
+    if (!b || mode < -1 || mode > 1) return PD_E_INVALID;
+    if (mode == 1 && !b->coef_ok) {
+        b->err = "the stored bond coefficients differ from their recomputation";
+        return PD_E_INVALID;
+    }
+    b->coef_mode = mode;
+    return PD_OK;
+}
+
+int pd_batch_coef_info(pd_batch *b, int *recomputed)
+{
+    if (!b || !recomputed) return PD_E_INVALID;
+    *recomputed = b->recompute();
     return PD_OK;
 }
 

@@ -227,3 +227,26 @@ def test_batch_separation_retires_fractured_plates():
     assert its1.sum() < its0.sum()
     for s in np.nonzero(ret1 == 0)[0]:
         assert np.array_equal(its0[:, s], its1[:, s]) and np.array_equal(u0[s], u1[s])
+
+
+def test_batch_recomputed_coefficients_match_streamed():
+    """The force phase recomputes each bond's {kappa, |xi|} from the reference
+    positions (default, after the set-up check) or streams them: the same
+    iterations, residuals, states and broken bonds, bit for bit, through
+    increments that break bonds; an unknown mode is refused."""
+    p = PB.plate_params()
+    specs = PB.plate_family(10, n_min=24, n_max=70)
+    with PB.Batch(p, specs) as rc, PB.Batch(p, specs) as st:
+        assert rc.coef_recomputed()
+        st.set_coef_mode(0)
+        assert not st.coef_recomputed()
+        for inc, du in enumerate([2.0e-6, 1.0e-5, 1.0e-5]):
+            a, ra, _ = rc.increment(du, 1e-6, 20000)
+            b, rb, _ = st.increment(du, 1e-6, 20000)
+            assert np.array_equal(a, b) and np.array_equal(ra, rb), inc
+            for s in range(len(specs)):
+                assert np.array_equal(rc.get(s, "U"), st.get(s, "U")), (inc, s)
+                assert np.array_equal(rc.csr(s)[2], st.csr(s)[2]), (inc, s)
+        assert sum(int((rc.csr(s)[2] == 0).sum()) for s in range(len(specs))) > 0
+        with pytest.raises(PdError):
+            st.set_coef_mode(2)
