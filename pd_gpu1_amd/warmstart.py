@@ -23,9 +23,11 @@ start also moves the free particles by a predicted increment:
   slowly; removing it is what lets the x correction help instead of hurt.
 
 ADR then converges from that guess to the same tolerance, so accuracy is
-unchanged and only the iteration count moves.  Fallback: a second handle of
-the same plate runs plain ADR from the plain initial state at the same time,
-and the first of the two runs to reach tol gives the increment.
+unchanged and only the iteration count moves.  Fallback: when the run from
+the guess does not reach tol, the increment restarts from the plain start;
+optionally a second handle of the same plate runs plain ADR at the same time
+and the first of the two runs to reach tol gives the increment
+(Plate.step_warm).
 
 Node features [N][16]: position relative to the plate centre / half side (2),
 du_prev / du (2), damage, top-strip flag, bottom-strip flag, 1, and eight
@@ -49,10 +51,10 @@ DU = 1.0e-6
 DX = 0.5e-3
 # Factor on the network's correction.  Chosen on validation plates (index
 # 128-191 of the evaluation family, disjoint from the plates the evaluation
-# reports): the full correction gives the lowest median iteration count but a
-# heavy tail (3.3x the uniform-stretch start's iterations on one plate in 64,
-# worse on 16); 0.75 was worse on 4 of 64, by 1.13x at most, and cut the
-# total by 21%.
+# reports): the full correction gives the lowest total and median iteration
+# count but a heavy tail (2.0x the uniform-stretch start's iterations on one
+# plate in 64, worse on 10); 0.75 was worse on 5 of 64, by 1.23x at most,
+# and cut the total by 20%.
 NET_SCALE = 0.75
 
 
