@@ -23,11 +23,13 @@ start also moves the free particles by a predicted increment:
   slowly; removing it is what lets the x correction help instead of hurt.
 
 ADR then converges from that guess to the same tolerance, so accuracy is
-unchanged and only the iteration count moves.  Fallback: when the run from
-the guess does not reach tol, the increment restarts from the plain start;
+unchanged and only the iteration count moves.  Fallback (Plate.step_warm):
+by default the configuration's rule, plain ADR for 50 iterations from the
+plain start against 50 iterations from the guess, the plain run taking over
+when the guess's residual is the larger (probe_then_solve), and a restart
+from the plain start when the run from the guess does not reach tol;
 optionally a second handle of the same plate runs plain ADR at the same time
-and the first of the two runs to reach tol gives the increment
-(Plate.step_warm).
+and the first of the two runs to reach tol gives the increment.
 
 Node features [N][16]: position relative to the plate centre / half side (2),
 du_prev / du (2), damage, top-strip flag, bottom-strip flag, 1, and eight
@@ -428,7 +430,7 @@ class Plate:
             self._mir, self._bmir = mir, bmir
         return self._mir, self._bmir
 
-    def step_warm(self, net, tol=1e-6, max_iters=400000, use_net=True, fallback=None, chunk=256):
+    def step_warm(self, net, tol=1e-6, max_iters=400000, use_net=True, fallback=None, chunk=256, probe=50):
         """Warm-started increment: the previous increment du_prev, or where
         uses_network says so the uniform-stretch increment plus (when use_net)
         the network's correction.  The problem is symmetric under the point
@@ -446,8 +448,13 @@ class Plate:
         stream, while this plate runs from the guess, both in chunks of
         `chunk` iterations on two host threads; the first to reach tol gives
         the increment (the other stops at its next chunk boundary) and the
-        result is this plate's state.  Returns (iterations, residual,
-        seconds, fell_back, predict seconds, used_net)."""
+        result is this plate's state.  Without `fallback`, the network's
+        guess is judged by the warm-start configuration's rule when `probe`
+        > 0 (default 50, probe_then_solve: plain ADR from the plain start and
+        ADR from the guess, `probe` iterations each, compared by residual),
+        every iteration counted; probe = 0 keeps only the restart.  Returns
+        (iterations, residual, seconds, fell_back, predict seconds,
+        used_net)."""
         t = time.perf_counter()
         u0 = self.prob.get("U")
         mask0 = self.prob.fail_mask()
@@ -476,6 +483,8 @@ class Plate:
             if fell_back:
                 self.prob.set("U", fallback.prob.get("U"))
                 self.prob.set_fail_mask(fallback.prob.fail_mask())
+        elif used_net and probe > 0:
+            its, r, fell_back = probe_then_solve(self.prob, guess, start, mask0, probe, tol, max_iters)
         else:
             self.prob.set("U", guess)
             its, r, ok = self.prob.adr_solve(0, max_iters, tol, nonfinite_ok=True)
@@ -491,6 +500,34 @@ class Plate:
         self.mask_prev = mask0
         self.k += 1
         return its, r, sec, fell_back, t_pred, used_net
+
+
+def probe_then_solve(prob, guess, start, mask0, probe, tol, max_iters):
+    """The fallback rule of the warm-start configuration: `probe` iterations
+    of plain ADR from the plain start, then `probe` iterations from the guess
+    (fail mask reset to mask0 before each run); when the guess's residual is
+    the larger (or not finite), plain ADR runs again from the plain start to
+    tol, otherwise the run from the guess continues to tol.  A probe that
+    reaches tol ends the increment.  Every iteration of every run is
+    counted.  Returns (iterations, residual, fell_back)."""
+    prob.set("U", start)
+    ip, rp, okp = prob.adr_solve(0, probe, tol, nonfinite_ok=True)
+    if okp:
+        return ip, rp, True
+    prob.set_fail_mask(mask0)
+    prob.set("U", guess)
+    iw, rw, okw = prob.adr_solve(0, probe, tol, nonfinite_ok=True)
+    if okw:
+        return ip + iw, rw, False
+    if rw <= rp:
+        i2, r2, ok2 = prob.adr_solve(iw, max_iters - iw, tol, nonfinite_ok=True)
+        if ok2:
+            return ip + iw + i2, r2, False
+        iw += i2
+    prob.set_fail_mask(mask0)
+    prob.set("U", start)
+    i3, r3, _ = prob.adr_solve(0, max_iters, tol)
+    return ip + iw + i3, r3, True
 
 
 def race(a, guess_a, b, guess_b, tol, max_iters, chunk=256):
@@ -531,11 +568,12 @@ def eval_specs(count, n_min=200, n_max=400, seed_base=0x5EED1004):
     return PB.plate_family(count, seed_base=seed_base, n_min=n_min, n_max=n_max, dx=DX)
 
 
-def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None, race_fallback=False):
+def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None, race_fallback=False, net_probe=50):
     """Plain ADR, the warm start without the network (du_prev, uniform stretch)
     and with it, on the same plates and increments (three handles per plate,
     each on its own load path; with race_fallback a fourth runs the network
-    path's plain-ADR fallback at the same time, see Plate.step_warm).  The
+    path's plain-ADR fallback at the same time; with net_probe > 0 the
+    network's guess is judged by probe_then_solve; see Plate.step_warm).  The
     network run's seconds include its prediction and its fallback.
     Returns per-increment rows (spec, k, its_plain, its_extrap, its_net, t_plain,
     t_extrap, t_net, fallback_extrap, fallback_net, used_net)."""
@@ -548,7 +586,8 @@ def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None, race_fallback=Fals
             for k in range(n_inc):
                 ip, rp_, rprobe, tp = plain.step_plain(tol, probe=probe)
                 ie, re_, te, fe, _, _ = extrap.step_warm(None, tol, use_net=False)
-                iw, rw, tw, fw, tpred, used = warm.step_warm(net, tol, use_net=True, fallback=shadow)
+                iw, rw, tw, fw, tpred, used = warm.step_warm(net, tol, use_net=True, fallback=shadow,
+                                                             probe=net_probe)
                 rows.append((s, k, ip, ie, iw, tp, te, tw, fe, fw, used))
                 if log:
                     log(f"[eval] plate {s} n={n} inc {k}: iterations plain {ip} no-net {ie} net {iw}  "
@@ -578,18 +617,21 @@ def main(argv=None):
     ap.add_argument("--first", type=int, default=0, help="index of the first plate of the seeded family")
     ap.add_argument("--increments", type=int, default=6)
     ap.add_argument("--race", action="store_true", help="run the plain-ADR fallback beside the network's start")
+    ap.add_argument("--probe", type=int, default=50,
+                    help="judge the network's start by the residual after this many iterations against plain ADR's "
+                         "(probe_then_solve; 0: restart fallback only)")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                       "mpnn_warmstart.npy"))
     a = ap.parse_args(argv)
     specs = eval_specs(a.first + a.plates)[a.first:]
     with Mpnn() as net:
         net.set_params(np.load(a.weights))
-        rows = np.array(evaluate(net, specs, a.increments, log=lambda m: print(m, flush=True), race_fallback=a.race),
-                        dtype=float)
+        rows = np.array(evaluate(net, specs, a.increments, log=lambda m: print(m, flush=True), race_fallback=a.race,
+                                 net_probe=a.probe), dtype=float)
     first, later = rows[:, 1] == 0, rows[:, 1] >= 1
     tot = lambda sel, c: float(rows[sel, c].sum())
     every = rows[:, 1] >= 0
-    out = {"plates": a.plates, "first": a.first, "race": a.race, "net_scale": NET_SCALE, "sides": [int(s[0]) for s in specs], "increments": a.increments,
+    out = {"plates": a.plates, "first": a.first, "race": a.race, "probe": a.probe, "net_scale": NET_SCALE, "sides": [int(s[0]) for s in specs], "increments": a.increments,
            "iterations": {"plain": tot(every, 2), "warm_no_network": tot(every, 3), "warm_network": tot(every, 4)},
            "seconds": {"plain": tot(every, 5), "warm_no_network": tot(every, 6), "warm_network": tot(every, 7)},
            "first_increment_iterations": {"plain": tot(first, 2), "uniform_stretch": tot(first, 3),

@@ -201,3 +201,45 @@ def test_epoch_order_is_a_permutation_sorted_by_side_within_windows():
     for k in range(0, 1000, 128):
         sides = [recs[i]["n"] for i in order[k:k + 128]]
         assert sides == sorted(sides)
+
+
+class _ProbeProb:
+    """Stands in for a handle: the residual after k updates from start `name`
+    is r0 * f**k; ADR stops once it is <= tol."""
+
+    def __init__(self, rates):
+        self.rates, self.cur, self.k, self.masks = rates, None, 0, 0
+
+    def set(self, field, u):
+        self.cur, self.k = u, 0
+
+    def set_fail_mask(self, mask):
+        self.masks += 1
+
+    def adr_solve(self, ct0, n, tol, nonfinite_ok=False):
+        assert ct0 == self.k  # a continuation carries on the iteration count
+        r0, f = self.rates[self.cur]
+        for i in range(n):
+            if r0 * f ** self.k <= tol:
+                return i, r0 * f ** self.k, True
+            self.k += 1
+        return n, r0 * f ** self.k, False
+
+
+def test_probe_rule_keeps_a_better_guess_and_counts_every_iteration():
+    prob = _ProbeProb({"plain": (1e-2, 0.99), "guess": (1e-3, 0.98)})
+    its, r, fell_back = W.probe_then_solve(prob, "guess", "plain", None, 50, 1e-6, 100000)
+    need = int(np.ceil(np.log(1e-3) / np.log(0.98)))  # updates from the guess to reach 1e-6
+    assert not fell_back and r <= 1e-6 and its == 50 + need and prob.cur == "guess"
+
+
+def test_probe_rule_falls_back_when_the_guess_residual_is_larger():
+    prob = _ProbeProb({"plain": (1e-2, 0.99), "guess": (1e-1, 0.995)})
+    its, r, fell_back = W.probe_then_solve(prob, "guess", "plain", None, 50, 1e-6, 100000)
+    need = int(np.ceil(np.log(1e-4) / np.log(0.99)))
+    assert fell_back and r <= 1e-6 and its == 50 + 50 + need and prob.cur == "plain"
+    assert prob.masks == 2  # the fail mask is reset before the guess's run and before the plain rerun
+    # a plain probe that converges ends the increment
+    prob = _ProbeProb({"plain": (1e-5, 0.5), "guess": (1.0, 0.5)})
+    its, r, fell_back = W.probe_then_solve(prob, "guess", "plain", None, 50, 1e-6, 100000)
+    assert fell_back and its == 4 and prob.cur == "plain"
