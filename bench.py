@@ -13,7 +13,10 @@ each LPS bond-update covering both sweeps); inputs are resident in HBM before
 the timed region starts.  The roofline object reports the dominant kernel
 (the LPS force pass): algorithmic bytes per launch / its mean HIP-event
 duration against the 8 TB/s HBM3E peak.  cpu_baseline times the FP64 oracle on
-the host cores over a bounded sub-block of the same workload.
+the host cores over a bounded sample of the same workload (its first 41
+steps), and its gpu_check compares the oracle's state after those steps with
+the GPU's: the FP64 path bit for bit, the mixed path at the timed launch
+shape by its error and broken bonds.
 
 The "adr" object measures the second half of the metric, ADR wall time to a
 1e-6 relative residual, on 2D PMB plates loaded in 1e-6 m strip increments:
@@ -107,9 +110,39 @@ def cpu_model():
     return {"cpu_model": name, "logical_cpus": os.cpu_count(), "cpus_available": avail}
 
 
-def cpu_baseline(sample_dims=(200, 200, 100), evals=40, one_thread_evals=2):
+def gpu_check(m, steps, shape):
+    """The oracle's state after `steps` steps against the GPU's after the same
+    steps: the FP64 path bit for bit, and the mixed path at the benchmark's
+    launch shape (the timed kernels) by its relative error and broken bonds."""
+    import numpy as np
+    from pd_gpu1_amd import _lib, autotune, scenarios
+
+    U = m.stack("U")
+    out = {"steps": steps}
+    for name, prec in (("fp64", _lib.PD_PREC_FP64), ("mixed", _lib.PD_PREC_MIXED)):
+        sc = scenarios.c2_lps_block(precision=prec)
+        prob = sc.make(device=0)
+        if prec == _lib.PD_PREC_MIXED:
+            autotune.apply(prob, shape)
+        prob.run_verlet(steps)
+        u = prob.get("U")
+        mask = prob.fail_mask()
+        prob.close()
+        if prec == _lib.PD_PREC_FP64:
+            out[name] = {"u_bit_exact": bool(np.array_equal(u, U)), "broken_bonds_equal": bool(np.array_equal(mask, m.fail))}
+        else:
+            out[name] = {"u_rel_l2": float(np.linalg.norm(u - U) / np.linalg.norm(U)),
+                         "u_rel_max": float(np.max(np.abs(u - U)) / np.max(np.abs(U))),
+                         "broken_bonds_differing": int((mask != m.fail).sum())}
+    out["oracle_broken_bonds"] = int((m.fail == 0).sum())
+    return out
+
+
+def cpu_baseline(sample_dims=(200, 200, 100), evals=40, one_thread_evals=2, check_shape=None):
     """FP64 oracle (OpenMP) on a sub-block of the same lattice and physics,
-    with all threads and with one (the reference's sequential CPU path)."""
+    with all threads and with one (the reference's sequential CPU path).
+    With check_shape (the full block only) the oracle's state after its timed
+    steps also checks the GPU's (gpu_check)."""
     import numpy as np
     from oracle import oracle as O
     from oracle.runner import OracleModel
@@ -128,6 +161,7 @@ def cpu_baseline(sample_dims=(200, 200, 100), evals=40, one_thread_evals=2):
     for _ in range(evals):
         m.step()
     dt = time.perf_counter() - t0
+    check = gpu_check(m, evals + 1, check_shape) if check_shape is not None and sample_dims == (200, 200, 100) else None
     threads = O.get_threads()
     O.set_threads(1)
     t0 = time.perf_counter()
@@ -146,6 +180,7 @@ def cpu_baseline(sample_dims=(200, 200, 100), evals=40, one_thread_evals=2):
         "one_thread": {"value": nb * one_thread_evals / dt1, "unit": "bond-updates/s",
                        "sample": f"the same, {one_thread_evals} steps on 1 thread, {dt1:.2f} s"},
         **cpu_model(),
+        **({"gpu_check": check} if check is not None else {}),
     }
 
 
@@ -653,7 +688,7 @@ def main():
                       if args.c3_full else adr_section(with_cpu=with_cpu))
         print(f"[bench] ADR measurements {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(check_shape=shape)
     print(json.dumps(out))
 
 
