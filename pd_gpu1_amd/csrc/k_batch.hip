@@ -141,10 +141,11 @@ __device__ __forceinline__ bool group_sync(const BatchGroups &gr, int grp, unsig
     return ok_sh != 0;
 }
 
+// The persistent kernel's body; k_batch_adr and k_batch_adr_dense below are
+// its two builds.
 template <int BT, int G, bool RC>
-__global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double tol, int max_iters,
-                                                  unsigned *queue, int32_t *iters_out, double *resid_out,
-                                                  BatchGroups gr)
+__device__ __forceinline__ void batch_adr(const BatchView &B, double du, double tol, int max_iters, unsigned *queue,
+                                          int32_t *iters_out, double *resid_out, const BatchGroups &gr)
 {
     __shared__ double sh[4][BT / 64];
     __shared__ int spec_sh, ok_sh;
@@ -429,6 +430,27 @@ __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double
 #undef PD_GROUP_SYNC
 }
 
+// One 1024-thread workgroup per CU at up to 128 VGPRs.
+template <int BT, int G, bool RC>
+__global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double tol, int max_iters,
+                                                  unsigned *queue, int32_t *iters_out, double *resid_out,
+                                                  BatchGroups gr)
+{
+    batch_adr<BT, G, RC>(B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+}
+
+// Two 1024-thread workgroups per CU: held at 64 VGPRs (the rest spills to
+// scratch, mostly outside the bond loop), twice the waves to hide the
+// latency of the bond chains when there are enough workgroups to fill them
+// (launches with at most kDenseParts workgroups per specimen).
+template <int G, bool RC>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void
+k_batch_adr_dense(BatchView B, double du, double tol, int max_iters, unsigned *queue, int32_t *iters_out,
+                  double *resid_out, BatchGroups gr)
+{
+    batch_adr<1024, G, RC>(B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+}
+
 } // namespace
 
 namespace {
@@ -458,11 +480,20 @@ __global__ void k_batch_coef_check(int n, const int32_t *__restrict__ rp, const 
 }
 
 template <int BT, bool RC>
-hipError_t batch_g(hipStream_t st, int grid, int group, const BatchView &B, double du, double tol, int max_iters,
-                   unsigned *queue, int32_t *iters_out, double *resid_out, const BatchGroups &gr)
+hipError_t batch_g(hipStream_t st, int grid, int group, bool dense, const BatchView &B, double du, double tol,
+                   int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out, const BatchGroups &gr)
 {
 #define PD_BATCH_CASE(GV)                                                                                    \
-    case GV: k_batch_adr<BT, GV, RC><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); break;
+    case GV:                                                                                                 \
+        if constexpr (BT == 1024) {                                                                          \
+            if (dense) {                                                                                     \
+                k_batch_adr_dense<GV, RC><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out,      \
+                                                               resid_out, gr);                               \
+                break;                                                                                       \
+            }                                                                                                \
+        }                                                                                                    \
+        k_batch_adr<BT, GV, RC><<<grid, BT, 0, st>>>(B, du, tol, max_iters, queue, iters_out, resid_out, gr); \
+        break;
     switch (group) {
     PD_BATCH_CASE(1) PD_BATCH_CASE(4) PD_BATCH_CASE(8) PD_BATCH_CASE(16) PD_BATCH_CASE(32)
     default: return hipErrorInvalidValue;
@@ -472,13 +503,14 @@ hipError_t batch_g(hipStream_t st, int grid, int group, const BatchView &B, doub
 }
 
 template <bool RC>
-hipError_t batch_b(hipStream_t st, int grid, int block, int group, const BatchView &B, double du, double tol,
-                   int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out, const BatchGroups &gr)
+hipError_t batch_b(hipStream_t st, int grid, int block, int group, bool dense, const BatchView &B, double du,
+                   double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out,
+                   const BatchGroups &gr)
 {
     switch (block) {
-    case 256: return batch_g<256, RC>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
-    case 512: return batch_g<512, RC>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
-    case 1024: return batch_g<1024, RC>(st, grid, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    case 256: return batch_g<256, RC>(st, grid, group, false, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    case 512: return batch_g<512, RC>(st, grid, group, false, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+    case 1024: return batch_g<1024, RC>(st, grid, group, dense, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
     default: return hipErrorInvalidValue;
     }
 }
@@ -500,16 +532,21 @@ hipError_t launch_batch_coef_check(hipStream_t st, int n, const int32_t *row_ptr
     return hipGetLastError();
 }
 
-// Workgroups of k_batch_adr<block, group, recompute> that fit on one CU at once.
-int batch_resident_per_cu(int block, int group, bool recompute)
+// Workgroups of the batched kernel (block, group, recompute; the dense build
+// for block 1024) that fit on one CU at once.
+int batch_resident_per_cu(int block, int group, bool recompute, bool dense)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
 #define PD_OCC(BT, GV)                                                                                      \
-    if (block == BT && group == GV)                                                                     \
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                               \
-            &n, recompute ? reinterpret_cast<const void *>(k_batch_adr<BT, GV, true>)                   \
-                          : reinterpret_cast<const void *>(k_batch_adr<BT, GV, false>), BT, 0);
+    if (block == BT && group == GV) {                                                                   \
+        const void *k = recompute ? reinterpret_cast<const void *>(k_batch_adr<BT, GV, true>)          \
+                                  : reinterpret_cast<const void *>(k_batch_adr<BT, GV, false>);         \
+        if (BT == 1024 && dense)                                                                        \
+            k = recompute ? reinterpret_cast<const void *>(k_batch_adr_dense<GV, true>)                 \
+                          : reinterpret_cast<const void *>(k_batch_adr_dense<GV, false>);               \
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, BT, 0);                                 \
+    }
     PD_OCC(256, 1) PD_OCC(256, 4) PD_OCC(256, 8) PD_OCC(256, 16) PD_OCC(256, 32)
     PD_OCC(512, 1) PD_OCC(512, 4) PD_OCC(512, 8) PD_OCC(512, 16) PD_OCC(512, 32)
     PD_OCC(1024, 1) PD_OCC(1024, 4) PD_OCC(1024, 8) PD_OCC(1024, 16) PD_OCC(1024, 32)
@@ -517,9 +554,9 @@ int batch_resident_per_cu(int block, int group, bool recompute)
     return e == hipSuccess && n > 0 ? n : 1;
 }
 
-hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
-                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out,
-                            const BatchGroups &gr)
+hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, bool dense, const BatchView &B,
+                            double du, double tol, int max_iters, unsigned *queue, int32_t *iters_out,
+                            double *resid_out, const BatchGroups &gr)
 {
     // W must divide the grid and the damping tree (block/64 waves)
     if (gr.parts < 1 || gr.parts > kMaxParts || (gr.parts & (gr.parts - 1)) || block / kMinPartLanes < gr.parts ||
@@ -528,6 +565,7 @@ hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, cons
         (B.recompute && !B.xy))
         return hipErrorInvalidValue;
     return B.recompute
-               ? batch_b<true>(st, grid, block, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr)
-               : batch_b<false>(st, grid, block, group, B, du, tol, max_iters, queue, iters_out, resid_out, gr);
+               ? batch_b<true>(st, grid, block, group, dense, B, du, tol, max_iters, queue, iters_out, resid_out, gr)
+               : batch_b<false>(st, grid, block, group, dense, B, du, tol, max_iters, queue, iters_out, resid_out,
+                                gr);
 }

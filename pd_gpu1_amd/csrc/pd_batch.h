@@ -67,7 +67,11 @@ hipError_t launch_pack_bond_pairs(hipStream_t st, int64_t nb, const double *kapp
                                   double2 *kd);
 hipError_t launch_batch_coef_check(hipStream_t st, int n, const int32_t *row_ptr, const int32_t *col,
                                    const double2 *kd, const double2 *xy, const BatchCoef &cg, unsigned *bad);
-int batch_resident_per_cu(int block, int group, bool recompute);
-hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, const BatchView &B, double du,
-                            double tol, int max_iters, unsigned *queue, int32_t *iters_out, double *resid_out,
-                            const BatchGroups &gr);
+// Launches with at most this many workgroups per specimen use the dense
+// build of the kernel (two 1024-thread workgroups per CU) when it is resident
+// more often than the default one.
+constexpr int kDenseParts = 4;
+int batch_resident_per_cu(int block, int group, bool recompute, bool dense);
+hipError_t launch_batch_adr(hipStream_t st, int grid, int block, int group, bool dense, const BatchView &B,
+                            double du, double tol, int max_iters, unsigned *queue, int32_t *iters_out,
+                            double *resid_out, const BatchGroups &gr);

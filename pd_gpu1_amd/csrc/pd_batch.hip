@@ -361,7 +361,15 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
     // every workgroup resident at once (a persistent grid): the capacity
     int dev_cu = 0;
     B_TRY(hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, b->device));
-    const int capacity = dev_cu * batch_resident_per_cu(block, b->group, b->recompute() != 0);
+    const bool rc = b->recompute() != 0;
+    const int capacity = dev_cu * batch_resident_per_cu(block, b->group, rc, false);
+    // The first launch of an increment runs the dense build (two 1024-thread
+    // workgroups per CU) when it gives each specimen at most kDenseParts
+    // workgroups; tail launches, which continue a few specimens with many
+    // workgroups each, run the default one.
+    const int capacity_dense = block == 1024 ? dev_cu * batch_resident_per_cu(block, b->group, rc, true) : 0;
+    const auto dense_for = [&](int w) { return capacity_dense > capacity && w <= kDenseParts; };
+    const auto cap_for = [&](int w) { return dense_for(w) ? capacity_dense : capacity; };
     int W = b->parts;
     const int wmax = std::min(kMaxFirstParts, block / 64);
     const int wtail = std::min(std::min(kMaxParts, b->tail), block / kMinPartLanes);
@@ -376,12 +384,16 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
             np[q] = b->spec[q].n;
         }
         W = pd_batch_choose_parts(nb.data(), np.data(), ret.data(), nspec, capacity, wmax);
+        if (dense_for(W)) {  // the rule again for the capacity of the dense build
+            const int Wd = pd_batch_choose_parts(nb.data(), np.data(), ret.data(), nspec, capacity_dense, wmax);
+            if (dense_for(Wd)) W = Wd;
+        }
     } else if (W > wmax) {
         b->err = "workgroups per specimen exceed block/64 (the damping tree's waves)";
         return PD_E_INVALID;
     }
-    int groups = grid > 0 ? grid / W : capacity / W;
-    if (W > 1 && groups > capacity / W) groups = capacity / W;  // a group's workgroups must all be resident
+    int groups = grid > 0 ? grid / W : cap_for(W) / W;
+    if (W > 1 && groups > cap_for(W) / W) groups = cap_for(W) / W;  // a group's workgroups must all be resident
     if (groups > nspec) groups = nspec;
     if (groups < 1) {
         b->err = "grid holds fewer workgroups than one specimen's group";
@@ -402,7 +414,7 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         if (rc) return rc;
     }
     // one persistent launch of `gw` groups of `w` workgroups over the view's queue
-    auto launch = [&](const BatchView &v, int w, int gw) -> int {
+    auto launch = [&](const BatchView &v, int w, int gw, bool dense) -> int {
         BatchGroups gr{w, nullptr, nullptr, nullptr, nullptr, nullptr, spin};
         if (w > 1) {
             int rc = ensure_groups(b, gw);
@@ -418,8 +430,8 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
             B_TRY(hipMemcpyAsync(b->d_active, &n_active, sizeof(int), hipMemcpyHostToDevice, b->stream));
             B_TRY(hipMemsetAsync(b->d_paused, 0, sizeof(int32_t) * nspec, b->stream));
         }
-        B_TRY(launch_batch_adr(b->stream, gw * w, block, b->group, v, du, tol, max_iters, b->queue, b->d_iters,
-                               b->d_resid, gr));
+        B_TRY(launch_batch_adr(b->stream, gw * w, block, b->group, dense, v, du, tol, max_iters, b->queue,
+                               b->d_iters, b->d_resid, gr));
         B_TRY(hipStreamSynchronize(b->stream));  // n_active lives on this frame
         if (w > 1) {
             int aborted = 0;
@@ -435,22 +447,26 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         return PD_OK;
     };
     // Tail launches: once every specimen has been handed out and at most
-    // half of the device's workgroups still run (pause_below specimens of w
+    // half of the launch's workgroups still run (pause_below specimens of w
     // workgroups), the running specimens pause at their next iteration and
-    // continue in a new launch with as many workgroups each as the device
-    // holds for them (at least twice as many, up to wtail), until w reaches
-    // wtail.
-    const auto pause_limit = [&](int w) { return tail && w < wtail ? std::max(1, capacity / (2 * w)) : 0; };
+    // continue in a new launch of the default build with as many workgroups
+    // each as the device holds for them (at least twice as many, up to
+    // wtail; after a dense launch the same number is enough), until w
+    // reaches wtail.
+    const auto pause_limit = [&](int w, bool dense) {
+        return tail && w < wtail ? std::max(1, (dense ? capacity_dense : capacity) / (2 * w)) : 0;
+    };
     HipEvents<2> ev;
     B_TRY(ev.create());
     hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     B_TRY(hipEventRecord(e0, b->stream));
     BatchView v = b->view();
-    v.pause_below = pause_limit(W);
+    bool dense = dense_for(W);
+    v.pause_below = pause_limit(W, dense);
     int w = W, gw = groups, tail_parts = 0;
     std::vector<int32_t> flags((size_t)nspec);
     for (;;) {
-        int rc = launch(v, w, gw);
+        int rc = launch(v, w, gw, dense);
         if (rc) return rc;
         if (v.pause_below == 0) break;
         B_TRY(copy_on(b->stream, flags.data(), b->d_paused, sizeof(int32_t) * nspec, hipMemcpyDeviceToHost));
@@ -468,8 +484,11 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         v.nqueue = (int)paused.size();
         v.resume = 1;
         v.it0 = b->d_iters;
-        v.pause_below = wn > w ? pause_limit(wn) : 0;  // a launch that cannot widen is the last
+        // a launch that cannot widen is the last (after a dense one, the
+        // default build at the same width may still widen later)
+        v.pause_below = wn > w || dense ? pause_limit(wn, false) : 0;
         w = wn;
+        dense = false;
         gw = std::min<int>((int)paused.size(), capacity / w);
         tail_parts = w;
     }
