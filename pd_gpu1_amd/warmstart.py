@@ -449,7 +449,7 @@ class Plate:
         `chunk` iterations on two host threads; the first to reach tol gives
         the increment (the other stops at its next chunk boundary) and the
         result is this plate's state.  Without `fallback`, the network's
-        guess is judged by the warm-start configuration's rule when `probe`
+        guess is checked by the warm-start configuration's rule when `probe`
         > 0 (default 50, probe_then_solve: plain ADR from the plain start and
         ADR from the guess, `probe` iterations each, compared by residual),
         every iteration counted; probe = 0 keeps only the restart.  Returns
@@ -573,7 +573,7 @@ def evaluate(net, specs, n_inc, tol=1e-6, probe=50, log=None, race_fallback=Fals
     and with it, on the same plates and increments (three handles per plate,
     each on its own load path; with race_fallback a fourth runs the network
     path's plain-ADR fallback at the same time; with net_probe > 0 the
-    network's guess is judged by probe_then_solve; see Plate.step_warm).  The
+    network's guess is checked by probe_then_solve; see Plate.step_warm).  The
     network run's seconds include its prediction and its fallback.
     Returns per-increment rows (spec, k, its_plain, its_extrap, its_net, t_plain,
     t_extrap, t_net, fallback_extrap, fallback_net, used_net)."""
@@ -618,7 +618,7 @@ def main(argv=None):
     ap.add_argument("--increments", type=int, default=6)
     ap.add_argument("--race", action="store_true", help="run the plain-ADR fallback beside the network's start")
     ap.add_argument("--probe", type=int, default=50,
-                    help="judge the network's start by the residual after this many iterations against plain ADR's "
+                    help="check the network's start by the residual after this many iterations against plain ADR's "
                          "(probe_then_solve; 0: restart fallback only)")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                       "mpnn_warmstart.npy"))
