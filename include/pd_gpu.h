@@ -245,8 +245,10 @@ int pd_batch_set_group(pd_batch *b, int group);
  * active work over the workgroups the device holds (the largest specimen
  * then no longer sets the wall time alone).  With 1024-thread workgroups and
  * W <= 4 an increment's first launch runs a build of the kernel that holds
- * two workgroups per CU (and W is chosen for that capacity); results are
- * the same.  The workgroups of a specimen
+ * two workgroups per CU (and W is chosen for that capacity); with a larger W
+ * and tail launches on (pd_batch_set_tail) the first launch takes W/2, as
+ * the tail launches widen the slowest specimens later.  Results are the
+ * same for every W.  The workgroups of a specimen
  * meet at barriers in global memory and are all resident (the grid is
  * capped); the damping sums keep the order of one workgroup, so results are
  * identical for every choice.  Other kernels running on the device at the

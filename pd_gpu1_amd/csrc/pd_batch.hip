@@ -387,6 +387,11 @@ int pd_batch_increment(pd_batch *b, double du, double tol, int max_iters, int bl
         if (dense_for(W)) {  // the rule again for the capacity of the dense build
             const int Wd = pd_batch_choose_parts(nb.data(), np.data(), ret.data(), nspec, capacity_dense, wmax);
             if (dense_for(Wd)) W = Wd;
+        } else if (b->tail && grid <= 0 && W > kDenseParts) {
+            // tail launches widen the slowest specimens later, so the first
+            // launch can start with half the workgroups per specimen and run
+            // twice the specimens at once
+            W /= 2;
         }
     } else if (W > wmax) {
         b->err = "workgroups per specimen exceed block/64 (the damping tree's waves)";
