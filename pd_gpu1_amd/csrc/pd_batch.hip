@@ -103,8 +103,9 @@ struct pd_batch {
 
 static void batch_free(pd_batch *b)
 {
-    void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->xy, b->fail, b->queue, b->d_iters,
-                    b->d_resid, b->d_retired, b->d_react, b->d_order2, b->snap_react, b->d_active, b->d_paused,
+    void *ptrs[] = {b->d_spec, b->d_order, b->buf, b->pos, b->row_ptr, b->col, b->kd, b->xy, b->fail, b->queue,
+                    b->d_iters, b->d_resid, b->d_retired, b->d_react, b->d_order2, b->snap_react, b->d_active,
+                    b->d_paused,
                     b->g_flag, b->g_bar, b->g_spec,
                     b->g_sum, b->g_abort, b->snap, b->snap_pos,
                     b->snap_fail, b->snap_ret};
