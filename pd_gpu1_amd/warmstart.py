@@ -622,8 +622,29 @@ def main(argv=None):
                          "(probe_then_solve; 0: restart fallback only)")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                       "mpnn_warmstart.npy"))
+    ap.add_argument("--eager-iterations", type=int, default=0,
+                    help="only run this many plain ADR iterations of the first plate's first increment, one "
+                         "pd_adr_iterate call each (no captured graph; for a per-kernel trace), and print the time")
     a = ap.parse_args(argv)
     specs = eval_specs(a.first + a.plates)[a.first:]
+    if a.eager_iterations > 0:
+        n, _, clen, ang = specs[0]
+        plate = Plate(n, clen, ang)
+        try:
+            plate.prob.strip_displace(DU, -DU)
+            plate.prob.adr_iterate(0)
+            plate.prob.sync()
+            t0 = time.perf_counter()
+            for ct in range(1, a.eager_iterations + 1):
+                plate.prob.adr_iterate(ct, sync=False)
+            c, r = plate.prob.adr_iterate(a.eager_iterations + 1)
+            sec = time.perf_counter() - t0
+        finally:
+            plate.close()
+        print(json.dumps({"plate": a.first, "side": n, "particles": len(plate.x), "bonds": len(plate.col),
+                          "eager_iterations": a.eager_iterations + 1, "us_per_iteration": sec * 1e6 /
+                          (a.eager_iterations + 1), "residual": r}), flush=True)
+        return 0
     with Mpnn() as net:
         net.set_params(np.load(a.weights))
         rows = np.array(evaluate(net, specs, a.increments, log=lambda m: print(m, flush=True), race_fallback=a.race,
