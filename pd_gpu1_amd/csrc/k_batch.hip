@@ -430,7 +430,8 @@ __device__ __forceinline__ void batch_adr(const BatchView &B, double du, double 
 #undef PD_GROUP_SYNC
 }
 
-// One 1024-thread workgroup per CU at up to 128 VGPRs.
+// The default build: up to 128 VGPRs, so one 1024-thread workgroup per CU
+// (or four of 256 threads, two of 512).
 template <int BT, int G, bool RC>
 __global__ __launch_bounds__(BT) void k_batch_adr(BatchView B, double du, double tol, int max_iters,
                                                   unsigned *queue, int32_t *iters_out, double *resid_out,
