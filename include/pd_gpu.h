@@ -368,11 +368,6 @@ int pd_set_tile(pd_handle *h, int tile_cells);
  * indices.  mode -1 = where every tile fits (default), 0 = never (per-bond
  * gathers from L2), 1 = required (PD_E_CAPACITY otherwise).  Same results. */
 int pd_set_lds(pd_handle *h, int mode);
-/* pd_run_verlet on LDS tiles (no counterpart in the reference): 1 = the force
- * kernel applies half 2 of each step and half 1 of the next to its rows as
- * they finish, alternating two particle-record buffers; 0 or -1 (default) =
- * separate particle passes.  Same results either way. */
-int pd_set_fused_verlet(pd_handle *h, int mode);
 /* Lanes per row of the FP64 PMB kernel: 1 (thread per row), 4, 8, 16, 32, or
  * -1 = 8 in 2D and 16 in 3D (default).  Terms are added in bond order, so the
  * forces and breaks are identical for every choice. */

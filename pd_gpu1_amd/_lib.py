@@ -29,7 +29,7 @@ EXPORTS = (
     "pd_adr_iterate", "pd_adr_solve", "pd_strip_displace", "pd_reduce_sum", "pd_reduce_sum_noalloc", "pd_damage",
     "pd_get_field", "pd_set_field", "pd_get_fail_bits", "pd_set_fail_bits", "pd_cylinder_init",
     "pd_cylinder_get", "pd_contact_step", "pd_save_text", "pd_checkpoint_write", "pd_checkpoint_read",
-    "pd_last_force_ms", "pd_time_force_passes", "pd_set_group", "pd_set_pass_group", "pd_stream_copy", "pd_set_tile", "pd_set_lds", "pd_set_lds_block", "pd_set_lds_pass_block", "pd_lds_info", "pd_set_coef_mode", "pd_coef_info", "pd_set_fp64_group", "pd_set_fused_verlet", "pd_sync",
+    "pd_last_force_ms", "pd_time_force_passes", "pd_set_group", "pd_set_pass_group", "pd_stream_copy", "pd_set_tile", "pd_set_lds", "pd_set_lds_block", "pd_set_lds_pass_block", "pd_lds_info", "pd_set_coef_mode", "pd_coef_info", "pd_set_fp64_group", "pd_sync",
     "pd_batch_create", "pd_batch_destroy", "pd_batch_last_error", "pd_batch_info", "pd_batch_spec_info",
     "pd_batch_increment", "pd_batch_set_group", "pd_batch_set_parts", "pd_batch_get_parts", "pd_batch_set_barrier_timeout", "pd_batch_choose_parts", "pd_batch_set_warm", "pd_batch_set_coef_mode", "pd_batch_coef_info", "pd_batch_set_retire", "pd_batch_set_tail", "pd_batch_get_tail_parts", "pd_batch_set_separation", "pd_batch_get_retired", "pd_batch_get_field", "pd_batch_set_field", "pd_batch_get_csr",
     "pd_mpnn_create", "pd_mpnn_destroy", "pd_mpnn_last_error", "pd_mpnn_num_params", "pd_mpnn_get_params",
@@ -126,7 +126,6 @@ def lib():
             "pd_set_coef_mode": [H, C.c_int],
             "pd_coef_info": [H, C.POINTER(C.c_int)],
             "pd_set_fp64_group": [H, C.c_int],
-            "pd_set_fused_verlet": [H, C.c_int],
             "pd_lds_info": [H, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "pd_set_pass_group": [H, C.c_int, C.c_int],
             "pd_stream_copy": [C.c_int, C.c_int64, C.c_int, D],

@@ -4,10 +4,10 @@
 // Tiles.  The tiled processing order groups the rows into small spatial
 // tiles (order[tile_start[t] .. tile_start[t+1])).  The staged set of a tile
 // is the sorted set of distinct neighbours of its rows; its particle records
-// are copied into LDS once per pass (four consecutive lanes move one 64-byte
+// are copied into LDS once per pass (two consecutive lanes move one 32-byte
 // record, so the copy is a coalesced stream of whole records), and every bond
 // of the tile then reads its neighbour from LDS through a 16-bit local index
-// lidx[k] instead of gathering 64 bytes from the L2 per bond.  Bit 15 of
+// lidx[k] instead of gathering 32 bytes from the L2 per bond.  Bit 15 of
 // lidx[k] mirrors the bond's fail bit (1 = intact), so one 2-byte load per
 // bond serves both; the fail words stay the bonds' state, breaks clear both,
 // and k_lidx_sync copies the fail words into bit 15 after any other change.
@@ -16,7 +16,10 @@
 //
 // Arithmetic, bond slots, the group reduction and the break decision are the
 // ones of the gathering kernels (mixed_bond.h) with the same bond-to-lane
-// assignment, so both paths produce the same bits.
+// assignment, so both paths produce the same bits.  Only the break band
+// differs: here one bound per row from the largest displacement of the
+// tile's staged records, there one per bond; either band decides every
+// break by the FP64 stretch wherever the FP32 one could err.
 #include "mixed_bond.h"
 
 #include <climits>
@@ -24,10 +27,6 @@
 namespace {
 
 constexpr int kHashSlots = 8192;  // per-tile open-addressing table of the builder
-// LDS record stride in 16-byte units: 80 bytes, so that neighbouring records
-// start on different bank groups (a 64-byte stride puts every record on the
-// same 4 of the 64 banks and serialises the 16-byte reads of a wave).
-constexpr int kLdsRec2 = 5;
 constexpr int kStageCap = 4096;   // largest staged set the builder handles (< 2^15)
 constexpr uint32_t kIntact = 0x8000u;  // bit 15 of a local index word
 
@@ -147,24 +146,33 @@ __global__ void k_lidx_sync(int64_t nb, const uint32_t *__restrict__ fail, uint1
 // Tiles handed to the 8 XCDs in contiguous ranges (gridDim.x is a multiple of 8).
 __device__ __forceinline__ int xcd_tile() { return xcd_block(); }
 
-// Copy the tile's staged records into LDS: 4 lanes per 64-byte record.
-__device__ __forceinline__ void stage_records(double2 *lds, const double *__restrict__ rec,
-                                              const int32_t *__restrict__ stage_idx, int64_t sb, int ns)
+// Copy the tile's staged records into LDS: 2 lanes per 32-byte record, the
+// record stride in LDS being the record itself.  With UMAX the largest
+// displacement component of the staged records is left in *umax (for the
+// break band of the tile's bonds).
+template <bool UMAX>
+__device__ __forceinline__ void stage_records(uint4 *lds, const uint32_t *__restrict__ rec,
+                                              const int32_t *__restrict__ stage_idx, int64_t sb, int ns,
+                                              unsigned *umax)
 {
-    for (int q = threadIdx.x; q < 4 * ns; q += blockDim.x) {
-        const int j = stage_idx[sb + (q >> 2)];
-        lds[kLdsRec2 * (q >> 2) + (q & 3)] = reinterpret_cast<const double2 *>(rec + (size_t)j * kRec)[q & 3];
+    float m = 0.f;
+    for (int q = threadIdx.x; q < 2 * ns; q += blockDim.x) {
+        const int j = stage_idx[sb + (q >> 1)];
+        const uint4 v = reinterpret_cast<const uint4 *>(rec + (size_t)j * kRecW)[q & 1];
+        lds[q] = v;
+        if (UMAX && (q & 1))
+            m = fmaxf(m, fmaxf(fmaxf(fabsf(__uint_as_float(v.x)), fabsf(__uint_as_float(v.y))),
+                               fabsf(__uint_as_float(v.z))));
+    }
+    if (UMAX) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0) atomicMax(umax, __float_as_uint(m));  // non-negative: ordered as integers
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ RecQ lds_rec(const double2 *lds, int l)
-{
-    const double2 *p = lds + kLdsRec2 * l;
-    const double2 a = p[0], b = p[1], d = p[2], e = p[3];
-    const float2 qr = __builtin_bit_cast(float2, b.y);
-    return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
-}
+__device__ __forceinline__ RecQ lds_rec(const uint4 *lds, int l) { return rec_of(lds[2 * l], lds[2 * l + 1]); }
 
 // 2-byte load at a 32-bit byte offset from a uniform base (no 64-bit
 // address arithmetic per bond; Nb < 2^31).
@@ -246,62 +254,51 @@ __device__ __forceinline__ uint32_t lidx_at(const uint16_t *lidx, int32_t k, int
         }                                                                                        \
     }
 
-#define PD_TILE_PROLOGUE                                                                         \
-    extern __shared__ double2 lds[];                                                             \
+// sbase: the tile's first entry in stage_idx; ut: largest displacement
+// component of its staged records (UMAX).
+#define PD_TILE_PROLOGUE(UMAX)                                                                   \
+    extern __shared__ uint4 lds[];                                                               \
+    __shared__ unsigned umax_sh;                                                                 \
     if (stop && *stop) return;                                                                   \
     const int t = xcd_tile();                                                                    \
     if (t >= ntiles) return;                                                                     \
-    stage_records(lds, rec, stage_idx, stage_ptr[t], (int)(stage_ptr[t + 1] - stage_ptr[t]));    \
+    if (UMAX && threadIdx.x == 0) umax_sh = 0u;                                                  \
+    if (UMAX) __syncthreads();                                                                   \
+    const int64_t sbase = stage_ptr[t];                                                          \
+    stage_records<UMAX>(lds, rec, stage_idx, sbase, (int)(stage_ptr[t + 1] - sbase), &umax_sh);  \
+    const float ut = UMAX ? __uint_as_float(umax_sh) : 0.f;                                      \
+    (void)ut;                                                                                    \
     const int lane = threadIdx.x & (G - 1);                                                      \
     const int r1 = tile_start[t + 1];
 
-// Verlet epilogue of a row whose force is final (FUSE kernels): lane d < DIM
-// of the row's group updates component d, ud being the row's u from its
-// record, with the arithmetic of k_verlet2 followed by k_verlet1, and writes
-// it to the next record buffer.  Rows past the integrated ones only move with
-// their strip (Solve_Plate_Crack_Grow.cu:20-27) or keep u.
-__device__ __forceinline__ void verlet_row(const VerletFuse &vf, int i, int d, float fd32, double ud)
-{
-    if (i < vf.N_int) {
-        double *vr = vf.vrec + (size_t)i * 8;
-        const double ad = ((double)fd32 + vr[3 + d]) / vf.dens;  // Integrate.cu:465-481
-        double vd = vr[d] + (vf.dt / 2.0) * ad;
-        vd = vd + (vf.dt / 2.0) * ad;                            // Integrate.cu:447-463
-        vr[d] = vd;
-        ud = ud + vd * vf.dt;
-    } else if (d == 1 && i < vf.N_int + vf.n_top + vf.n_bot) {
-        ud = i < vf.N_int + vf.n_top ? ud + vf.load * vf.dt : ud - vf.load * vf.dt;
-    }
-    vf.rec_next[(size_t)i * kRec + 4 + d] = ud;
-}
-
-template <int DIM, int G, bool GC, bool FUSE>
+template <int DIM, int G, bool GC>
 __device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int ntiles,
-                                                  const int32_t *__restrict__ tile_start,
-                                                  const int32_t *__restrict__ order,
-                                                  const int64_t *__restrict__ stage_ptr,
-                                                  const int32_t *__restrict__ stage_idx,
-                                                  const int32_t *__restrict__ rp, uint16_t *lidx,
-                                                  const float2 *__restrict__ bnd, uint32_t *fail,
-                                                  const double *__restrict__ rec, F3 f, double s0, float s0f,
-                                                  float win, double ymin, GeomCoef gc, VerletFuse vf)
+                                             const int32_t *__restrict__ tile_start,
+                                             const int32_t *__restrict__ order,
+                                             const int64_t *__restrict__ stage_ptr,
+                                             const int32_t *__restrict__ stage_idx,
+                                             const int32_t *__restrict__ rp, uint16_t *lidx,
+                                             const float2 *__restrict__ bnd, uint32_t *fail,
+                                             const uint32_t *__restrict__ rec, F3 f, double s0, const MixCtx &mc,
+                                             double ymin, GeomCoef gc)
 {
-    PD_TILE_PROLOGUE
+    PD_TILE_PROLOGUE(true)
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
         const int i = order[r];
         const Rec ri = load_rec(rec, i).r;
+        const float win = bond_window(mc, umag(ri), ut);
         double ax = 0.0, ay = 0.0, az = 0.0;
 #define PMB_BODY(R, S, K)                                                                        \
     {                                                                                            \
-        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                          \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<false>(g.xi2, gc, L0, cf, rL0);                                        \
-        const float s = g.num * frcp(L0 * g.len_sum(L0));                                           \
-        const float tb = masked(cf * s * g.rL, S.mask);                                      \
+        const float s = g.num * frcp(L0 * g.len_sum(L0));                                        \
+        const float tb = masked(cf * s * g.rL, S.mask);                                          \
         ax += (double)(tb * g.dx);                                                               \
         ay += (double)(tb * g.dy);                                                               \
         if (DIM == 3) az += (double)(tb * g.dz);                                                 \
-        check_break<DIM, false>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail, lidx);               \
+        check_break<DIM, false, true>(i, S.c, stage_idx, sbase, s, S, K, win, mc, s0, ymin, fail, lidx); \
     }
         PD_LDS_ROW_WALK_PF2(true, PMB_BODY)
 #undef PMB_BODY
@@ -312,8 +309,6 @@ __device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int n
             f.p[1][i] = sy;
             if (DIM == 3) f.p[2][i] = sz;
         }
-        if (FUSE && lane < DIM)
-            verlet_row(vf, i, lane, lane == 0 ? sx : lane == 1 ? sy : sz, lane == 0 ? ri.u0 : lane == 1 ? ri.u1 : ri.u2);
     }
 }
 
@@ -325,25 +320,26 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
                                                         const int32_t *__restrict__ stage_idx,
                                                         const int32_t *__restrict__ rp,
                                                         const uint16_t *__restrict__ lidx,
-                                                        const float2 *__restrict__ bnd, double *rec,
+                                                        const float2 *__restrict__ bnd, uint32_t *rec,
                                                         const float *__restrict__ minv32, float ctheta,
                                                         int skip_broken, float *theta32, GeomCoef gc)
 {
-    PD_TILE_PROLOGUE
+    PD_TILE_PROLOGUE(false)
     const bool need_intact = skip_broken != 0;
     // theta/m of this pass is written into the records after every tile has
-    // staged them: the x0 half (doubles 0..3) is read here, the writes go to
-    // double 3 of rows this tile owns, and the next pass reads them.
+    // staged them: the position and displacement words are read here, the
+    // writes go to words 3 and 7 of rows this tile owns, and the next pass
+    // reads them.
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
         const int i = order[r];
         const Rec ri = load_rec(rec, i).r;
         double acc = 0.0;
 #define THETA_BODY(R, S, K)                                                                      \
     {                                                                                            \
-        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                             \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
-        const float e = g.num * frcp(g.len_sum(L0)); /* |xi+eta| - |xi| */                            \
+        const float e = g.num * frcp(g.len_sum(L0)); /* |xi+eta| - |xi| */                       \
         acc += (double)masked(cf * L0 * e, S.mask);                                              \
     }
         PD_LDS_ROW_WALK(need_intact, THETA_BODY)
@@ -353,41 +349,52 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
             const float rm = minv32[i];
             const float theta = ctheta * rm * th;
             theta32[i] = theta;
-            *reinterpret_cast<float2 *>(rec + (size_t)i * kRec + 3) = make_float2(theta * rm, rm);
+            rec[(size_t)i * kRecW + 3] = __float_as_uint(theta * rm);
+            rec[(size_t)i * kRecW + 7] = __float_as_uint(rm);
         }
     }
 }
 
-template <int DIM, int G, bool GC, bool FUSE>
-__device__ __forceinline__ void lps_force_lds_body(const int *__restrict__ stop, int ntiles,
-                                                        const int32_t *__restrict__ tile_start,
-                                                        const int32_t *__restrict__ order,
-                                                        const int64_t *__restrict__ stage_ptr,
-                                                        const int32_t *__restrict__ stage_idx,
-                                                        const int32_t *__restrict__ rp, uint16_t *lidx,
-                                                        const float2 *__restrict__ bnd, uint32_t *fail,
-                                                        const double *__restrict__ rec, float A, float B,
-                                                        double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc,
-                                                        VerletFuse vf)
+// amdgpu_waves_per_eu(6): at most 80 VGPRs, so that three 512-thread
+// workgroups share a CU (with 64-byte records this measured 1.10 ms instead
+// of 1.26 ms per LPS force pass on the 4M-particle block).
+template <int DIM, int G, bool GC>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1))) void k_pmb_lds(
+    const int *__restrict__ stop, int ntiles, const int32_t *__restrict__ tile_start,
+    const int32_t *__restrict__ order, const int64_t *__restrict__ stage_ptr, const int32_t *__restrict__ stage_idx,
+    const int32_t *__restrict__ rp, uint16_t *lidx, const float2 *__restrict__ bnd, uint32_t *fail,
+    const uint32_t *__restrict__ rec, F3 f, double s0, MixCtx mc, double ymin, GeomCoef gc)
 {
-    PD_TILE_PROLOGUE
+    pmb_lds_body<DIM, G, GC>(stop, ntiles, tile_start, order, stage_ptr, stage_idx, rp, lidx, bnd, fail, rec, f, s0,
+                             mc, ymin, gc);
+}
+
+template <int DIM, int G, bool GC>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1))) void k_lps_force_lds(
+    const int *__restrict__ stop, int ntiles, const int32_t *__restrict__ tile_start,
+    const int32_t *__restrict__ order, const int64_t *__restrict__ stage_ptr, const int32_t *__restrict__ stage_idx,
+    const int32_t *__restrict__ rp, uint16_t *lidx, const float2 *__restrict__ bnd, uint32_t *fail,
+    const uint32_t *__restrict__ rec, float A, float B, double s0, MixCtx mc, double ymin, F3 f, GeomCoef gc)
+{
+    PD_TILE_PROLOGUE(true)
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
         const int i = order[r];
         const RecQ qi = load_rec(rec, i);
         const Rec &ri = qi.r;
+        const float win = bond_window(mc, umag(ri), ut);
         double ax = 0.0, ay = 0.0, az = 0.0;
 #define LPSF_BODY(R, S, K)                                                                       \
     {                                                                                            \
-        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                                \
+        const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                          \
         float L0 = S.L0, cf = S.coef, rL0 = 0.f;                                                 \
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
-        const float e = g.num * frcp(g.len_sum(L0));                                                  \
+        const float e = g.num * frcp(g.len_sum(L0));                                             \
         const float s = e * (GC ? rL0 : frcp(L0));                                               \
         const float tt = masked(cf * __builtin_fmaf(A * L0, qi.q + R.q, B * e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
-        check_break<DIM, true>(ri, R.r, s, S, K, s0f, win, s0, ymin, fail, lidx);                \
+        check_break<DIM, true, true>(i, S.c, stage_idx, sbase, s, S, K, win, mc, s0, ymin, fail, lidx); \
     }
         PD_LDS_ROW_WALK(true, LPSF_BODY)
 #undef LPSF_BODY
@@ -398,63 +405,8 @@ __device__ __forceinline__ void lps_force_lds_body(const int *__restrict__ stop,
             f.p[1][i] = sy;
             if (DIM == 3) f.p[2][i] = sz;
         }
-        if (FUSE && lane < DIM)
-            verlet_row(vf, i, lane, lane == 0 ? sx : lane == 1 ? sy : sz, lane == 0 ? ri.u0 : lane == 1 ? ri.u1 : ri.u2);
     }
 }
-
-// The force kernels, and their variants that also apply the Verlet epilogue
-// (separate names, so that profiles tell the two apart).
-#define PD_PMB_LDS_PARAMS                                                                                   \
-    const int *__restrict__ stop, int ntiles, const int32_t *__restrict__ tile_start,                      \
-        const int32_t *__restrict__ order, const int64_t *__restrict__ stage_ptr,                          \
-        const int32_t *__restrict__ stage_idx, const int32_t *__restrict__ rp, uint16_t *lidx,             \
-        const float2 *__restrict__ bnd, uint32_t *fail, const double *__restrict__ rec, F3 f, double s0,  \
-        float s0f, float win, double ymin, GeomCoef gc, VerletFuse vf
-#define PD_PMB_LDS_ARGS \
-    stop, ntiles, tile_start, order, stage_ptr, stage_idx, rp, lidx, bnd, fail, rec, f, s0, s0f, win, ymin, gc, vf
-#define PD_LPSF_LDS_PARAMS                                                                                  \
-    const int *__restrict__ stop, int ntiles, const int32_t *__restrict__ tile_start,                      \
-        const int32_t *__restrict__ order, const int64_t *__restrict__ stage_ptr,                          \
-        const int32_t *__restrict__ stage_idx, const int32_t *__restrict__ rp, uint16_t *lidx,             \
-        const float2 *__restrict__ bnd, uint32_t *fail, const double *__restrict__ rec, float A, float B,  \
-        double s0, float s0f, float win, double ymin, F3 f, GeomCoef gc, VerletFuse vf
-#define PD_LPSF_LDS_ARGS \
-    stop, ntiles, tile_start, order, stage_ptr, stage_idx, rp, lidx, bnd, fail, rec, A, B, s0, s0f, win, ymin, f, gc, vf
-
-// amdgpu_waves_per_eu(6): at most 80 VGPRs, so that three 512-thread
-// workgroups with 4 dx tiles (about 49 KB of staged records each) share a CU;
-// measured 1.10 ms instead of 1.26 ms per LPS force pass on the 4M-particle
-// block (the tuner then picks 8 lanes per row).
-template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1))) void k_pmb_lds(PD_PMB_LDS_PARAMS)
-{
-    pmb_lds_body<DIM, G, GC, false>(PD_PMB_LDS_ARGS);
-}
-
-template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) void k_pmb_verlet_lds(PD_PMB_LDS_PARAMS)
-{
-    pmb_lds_body<DIM, G, GC, true>(PD_PMB_LDS_ARGS);
-}
-
-template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1))) void k_lps_force_lds(PD_LPSF_LDS_PARAMS)
-{
-    lps_force_lds_body<DIM, G, GC, false>(PD_LPSF_LDS_ARGS);
-}
-
-template <int DIM, int G, bool GC>
-__global__ __launch_bounds__(1024) void k_lps_force_verlet_lds(PD_LPSF_LDS_PARAMS)
-{
-    lps_force_lds_body<DIM, G, GC, true>(PD_LPSF_LDS_ARGS);
-}
-
-template <int DIM, int G, bool GC, bool FUSE>
-struct ForceKernels {
-    static constexpr auto pmb = FUSE ? k_pmb_verlet_lds<DIM, G, GC> : k_pmb_lds<DIM, G, GC>;
-    static constexpr auto lpsf = FUSE ? k_lps_force_verlet_lds<DIM, G, GC> : k_lps_force_lds<DIM, G, GC>;
-};
 
 inline unsigned tiles_grid(int ntiles) { return (unsigned)((ntiles + 7) / 8 * 8); }
 
@@ -475,35 +427,6 @@ hipError_t lds_attr(K kernel, size_t bytes)
     default: return hipErrorInvalidValue;                 \
     }
 
-// force kernels: also the Verlet epilogue (vf != nullptr) or not
-#define PD_FSWITCH(FN, DIM, G, GC, GCV, VF, ...)                                        \
-    if (VF) {                                                                           \
-        switch (G) {                                                                    \
-        case 4: return FN<DIM, 4, GC, true>(__VA_ARGS__, GCV, *(VF));                   \
-        case 8: return FN<DIM, 8, GC, true>(__VA_ARGS__, GCV, *(VF));                   \
-        case 16: return FN<DIM, 16, GC, true>(__VA_ARGS__, GCV, *(VF));                 \
-        case 32: return FN<DIM, 32, GC, true>(__VA_ARGS__, GCV, *(VF));                 \
-        case 64: return FN<DIM, 64, GC, true>(__VA_ARGS__, GCV, *(VF));                 \
-        default: return hipErrorInvalidValue;                                           \
-        }                                                                               \
-    } else {                                                                            \
-        switch (G) {                                                                    \
-        case 4: return FN<DIM, 4, GC, false>(__VA_ARGS__, GCV, VerletFuse{});           \
-        case 8: return FN<DIM, 8, GC, false>(__VA_ARGS__, GCV, VerletFuse{});           \
-        case 16: return FN<DIM, 16, GC, false>(__VA_ARGS__, GCV, VerletFuse{});         \
-        case 32: return FN<DIM, 32, GC, false>(__VA_ARGS__, GCV, VerletFuse{});         \
-        case 64: return FN<DIM, 64, GC, false>(__VA_ARGS__, GCV, VerletFuse{});         \
-        default: return hipErrorInvalidValue;                                           \
-        }                                                                               \
-    }
-
-#define PD_DISPATCH_FORCE(FN, DIM, G, GCP, VF, ...)                          \
-    if (GCP) {                                                               \
-        PD_FSWITCH(FN, DIM, G, true, *(GCP), VF, __VA_ARGS__)                \
-    } else {                                                                 \
-        PD_FSWITCH(FN, DIM, G, false, GeomCoef{}, VF, __VA_ARGS__)           \
-    }
-
 // gc != nullptr: recompute the bond coefficients from |xi| (no coefficient stream)
 #define PD_DISPATCH_LDS(FN, DIM, G, GCP, ...)                                \
     if (GCP) {                                                               \
@@ -512,24 +435,23 @@ hipError_t lds_attr(K kernel, size_t bytes)
         PD_LSWITCH(FN, DIM, G, false, GeomCoef{}, __VA_ARGS__)               \
     }
 
-template <int DIM, int G, bool GC, bool FUSE>
+template <int DIM, int G, bool GC>
 hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
-                     const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0, double window,
-                     double ymin, GeomCoef gc, VerletFuse vf)
+                     const float2 *bnd, uint32_t *fail, const uint32_t *rec, F3 f, double s0, const MixCtx &mc,
+                     double ymin, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
-    constexpr auto kern = ForceKernels<DIM, G, GC, FUSE>::pmb;
-    hipError_t e = lds_attr(kern, bytes);
+    hipError_t e = lds_attr(k_pmb_lds<DIM, G, GC>, bytes);
     if (e != hipSuccess) return e;
-    kern<<<tiles_grid(T.ntiles), T.block, bytes, s>>>(
-        stop, T.ntiles, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, f, s0, (float)s0,
-        stretch_window(s0, window), ymin, gc, vf);
+    k_pmb_lds<DIM, G, GC><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order,
+                                                                      T.stage_ptr, T.stage_idx, rp, T.lidx, bnd,
+                                                                      fail, rec, f, s0, mc, ymin, gc);
     return hipGetLastError();
 }
 
 template <int DIM, int G, bool GC>
 hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
-                       const float2 *bnd, double *rec, const float *minv32, float ctheta,
+                       const float2 *bnd, uint32_t *rec, const float *minv32, float ctheta,
                        int skip_broken, float *theta32, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
@@ -542,18 +464,17 @@ hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const 
     return hipGetLastError();
 }
 
-template <int DIM, int G, bool GC, bool FUSE>
+template <int DIM, int G, bool GC>
 hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
-                      const float2 *bnd, uint32_t *fail, const double *rec, float A, float B, double s0,
-                      double window, double ymin, F3 f, GeomCoef gc, VerletFuse vf)
+                      const float2 *bnd, uint32_t *fail, const uint32_t *rec, float A, float B, double s0,
+                      const MixCtx &mc, double ymin, F3 f, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
-    constexpr auto kern = ForceKernels<DIM, G, GC, FUSE>::lpsf;
-    hipError_t e = lds_attr(kern, bytes);
+    hipError_t e = lds_attr(k_lps_force_lds<DIM, G, GC>, bytes);
     if (e != hipSuccess) return e;
-    kern<<<tiles_grid(T.ntiles_int), T.block2, bytes, s>>>(
-        stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0,
-        (float)s0, stretch_window(s0, window), ymin, f, gc, vf);
+    k_lps_force_lds<DIM, G, GC><<<tiles_grid(T.ntiles_int), T.block2, bytes, s>>>(
+        stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0, mc,
+        ymin, f, gc);
     return hipGetLastError();
 }
 
@@ -576,46 +497,43 @@ hipError_t launch_lidx_sync(hipStream_t s, int64_t nb, const uint32_t *fail, uin
 }
 
 int stage_cap() { return kStageCap; }
-int lds_record_bytes() { return 16 * kLdsRec2; }
+int lds_record_bytes() { return 4 * kRecW; }
 
 hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
-                          const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
-                          double window, double ymin, const GeomCoef *gc, const VerletFuse *vf)
+                          const int32_t *rp, const float2 *bnd, uint32_t *fail, const uint32_t *rec, F3 f, double s0,
+                          const MixCtx &mc, double ymin, const GeomCoef *gc)
 {
     if (T.ntiles <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_FORCE(pmb_lds_g, 3, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_LDS(pmb_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, f, s0, mc, ymin)
     } else {
-        PD_DISPATCH_FORCE(pmb_lds_g, 2, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, f, s0, window, ymin)
+        PD_DISPATCH_LDS(pmb_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, f, s0, mc, ymin)
     }
 }
 
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
-                                const int32_t *order, const int32_t *rp, const float2 *bnd,
-                                double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32,
-                                const GeomCoef *gc)
+                                const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *rec,
+                                const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc)
 {
     if (T.ntiles <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(theta_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta,
-                        skip_broken, theta32)
+        PD_DISPATCH_LDS(theta_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta, skip_broken,
+                        theta32)
     } else {
-        PD_DISPATCH_LDS(theta_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta,
-                        skip_broken, theta32)
+        PD_DISPATCH_LDS(theta_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta, skip_broken,
+                        theta32)
     }
 }
 
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
-                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f,
-                                const GeomCoef *gc, const VerletFuse *vf)
+                                const uint32_t *rec, float A, float B, double s0, const MixCtx &mc, double ymin, F3 f,
+                                const GeomCoef *gc)
 {
     if (T.ntiles_int <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_FORCE(lpsf_lds_g, 3, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin,
-                          f)
+        PD_DISPATCH_LDS(lpsf_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, mc, ymin, f)
     } else {
-        PD_DISPATCH_FORCE(lpsf_lds_g, 2, group, gc, vf, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, window, ymin,
-                          f)
+        PD_DISPATCH_LDS(lpsf_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, mc, ymin, f)
     }
 }

@@ -16,14 +16,15 @@
 
 namespace {
 
-// Mixed-precision force kernels gather u from the displacement half of a
-// 64-byte particle record (doubles 4..7: u_x, u_y, u_z, 0).  Kernels that move
-// particles refresh that half in the same pass when given the record array.
-__device__ __forceinline__ void store_rec_u(double *rec, int i, double ux, double uy, double uz)
+// Mixed-precision force kernels gather u, rounded to FP32 and in units of
+// the record quantum, from words 4..6 of a particle record (kRecW words).
+// Kernels that move particles refresh those words in the same pass when
+// given the record array.
+__device__ __forceinline__ void store_rec_u(const RecU &ru, int i, double ux, double uy, double uz)
 {
-    double2 *r = reinterpret_cast<double2 *>(rec + (size_t)i * 8 + 4);
-    r[0] = make_double2(ux, uy);
-    r[1] = make_double2(uz, 0.0);
+    uint32_t *r = ru.rec + (size_t)i * kRecW + 4;
+    *reinterpret_cast<float2 *>(r) = make_float2((float)(ux * ru.iq), (float)(uy * ru.iq));
+    r[2] = __float_as_uint((float)(uz * ru.iq));
 }
 
 
@@ -95,7 +96,7 @@ __device__ __forceinline__ bool finish_tree(int G, double *partials, Vec<NC> &to
 
 template <int DIM>
 __global__ void k_verlet1(int N_int, int n_top, int n_bot, double dt, double load, D3 v, CD3 a, D3 u,
-                          double *cyl, double *rec)
+                          double *cyl, RecU rec)
 {
     // Integrate.cu:447-463 (i < N_int) and Solve_Plate_Crack_Grow.cu:20-27
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,7 +126,7 @@ __global__ void k_verlet1(int N_int, int n_top, int n_bot, double dt, double loa
         ud[1] = i < N_int + n_top ? ud[1] + load * dt : ud[1] - load * dt;
         u.p[1][i] = ud[1];
     }
-    if (rec) store_rec_u(rec, i, ud[0], ud[1], ud[2]);
+    if (rec.rec) store_rec_u(rec, i, ud[0], ud[1], ud[2]);
 }
 
 // Half 2 of one step and half 1 of the next in one pass (runs without the
@@ -135,7 +136,7 @@ __global__ void k_verlet1(int N_int, int n_top, int n_bot, double dt, double loa
 // the last step's k_verlet2 stores it.
 template <int DIM, bool F32>
 __global__ void k_verlet21(int N_int, int n_top, int n_bot, double dt, double dens, double load, D3 v, CD3 f,
-                           CF3 f32, CD3 b, D3 u, double *rec)
+                           CF3 f32, CD3 b, D3 u, RecU rec)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N_int + n_top + n_bot) return;
@@ -161,48 +162,7 @@ __global__ void k_verlet21(int N_int, int n_top, int n_bot, double dt, double de
         ud[1] = i < N_int + n_top ? ud[1] + load * dt : ud[1] - load * dt;
         u.p[1][i] = ud[1];
     }
-    if (rec) store_rec_u(rec, i, ud[0], ud[1], ud[2]);
-}
-
-// Particles the fused LPS force pass does not visit (i >= N_int): the strip
-// motion of half 1 (Solve_Plate_Crack_Grow.cu:20-27) and the u half of the
-// next record buffer.
-__global__ void k_verlet21_tail(int dim, int N_int, int N, int n_top, int n_bot, double dt, double load,
-                                const double *__restrict__ rec_cur, double *rec_next)
-{
-    const int i = N_int + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= N) return;
-    const double *r = rec_cur + (size_t)i * 8 + 4;
-    double uy = r[1];
-    if (i < N_int + n_top)
-        uy += load * dt;
-    else if (i < N_int + n_top + n_bot)
-        uy -= load * dt;
-    store_rec_u(rec_next, i, r[0], uy, dim == 3 ? r[2] : 0.0);
-}
-
-// {v, b} records of a fused Verlet run, and the SoA u and v after it.
-__global__ void k_pack_vrec(int dim, int N, CD3 v, CD3 b, double *vrec)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    double2 *r = reinterpret_cast<double2 *>(vrec + (size_t)i * 8);
-    r[0] = make_double2(v.p[0][i], v.p[1][i]);
-    r[1] = make_double2(dim == 3 ? v.p[2][i] : 0.0, b.p[0][i]);
-    r[2] = make_double2(b.p[1][i], dim == 3 ? b.p[2][i] : 0.0);
-    r[3] = make_double2(0.0, 0.0);
-}
-
-__global__ void k_unpack_uv(int dim, int N, const double *__restrict__ rec, const double *__restrict__ vrec, D3 u,
-                            D3 v)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const double *r = rec + (size_t)i * 8 + 4, *w = vrec + (size_t)i * 8;
-    for (int d = 0; d < dim; ++d) {
-        u.p[d][i] = r[d];
-        v.p[d][i] = w[d];
-    }
+    if (rec.rec) store_rec_u(rec, i, ud[0], ud[1], ud[2]);
 }
 
 template <int DIM, bool F32>
@@ -334,7 +294,7 @@ __global__ void k_adr_finish(int G, double *partials, DevScalars *S, double tol)
 }
 
 __global__ void k_adr_update(int dim, int N_int, double mass, double dt, CD3 f, CF3 f32, int use32, CD3 b,
-                             D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S, double *rec)
+                             D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S, RecU rec)
 {
     // kernel_integrate, Integrate.cu:62-105
     if (S->done) return;
@@ -355,7 +315,7 @@ __global__ void k_adr_update(int dim, int N_int, double mass, double dt, CD3 f, 
         vho.p[d][i] = hv;
         fo.p[d][i] = fd;
     }
-    if (rec) store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
+    if (rec.rec) store_rec_u(rec, i, u.p[0][i], u.p[1][i], dim == 3 ? u.p[2][i] : 0.0);
 }
 
 __global__ void k_set_ct(DevScalars *S, int ct, int clear_done)
@@ -374,7 +334,7 @@ __global__ void k_set_ct(DevScalars *S, int ct, int clear_done)
 // (Solve_Kalthoff_Winkler.cu:168-170); k_contact_sum adds the terms in the
 // order of the fixed T-lane tree.
 __global__ void k_contact_terms(int N, const double *__restrict__ dmg, CD3 x, D3 u, D3 v, pd_cyl cyl, double dt,
-                                double dens, double vol, const DevScalars *S, double *terms, double *rec)
+                                double dens, double vol, const DevScalars *S, double *terms, RecU rec)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -409,7 +369,7 @@ __global__ void k_contact_terms(int N, const double *__restrict__ dmg, CD3 x, D3
             u.p[0][i] = nux;
             u.p[1][i] = nuy;
             u.p[2][i] = nuz;
-            if (rec) store_rec_u(rec, i, nux, nuy, nuz);
+            if (rec.rec) store_rec_u(rec, i, nux, nuy, nuz);
             double nvx = (nux - oux) / dt, nvy = (nuy - ouy) / dt, nvz = (nuz - ouz) / dt;
             v.p[0][i] = nvx;
             v.p[1][i] = nvy;
@@ -516,7 +476,7 @@ inline int nblk(int64_t n, int t) { return n > 0 ? (int)((n + t - 1) / t) : 1; }
 } // namespace
 
 hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
-                          D3 v, CD3 a, D3 u, double *cyl_state, double *rec)
+                          D3 v, CD3 a, D3 u, double *cyl_state, RecU rec)
 {
     int n = N_int + n_top + n_bot;
     if (n < 1) n = 1;
@@ -527,28 +487,8 @@ hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bo
     return hipGetLastError();
 }
 
-hipError_t launch_verlet21_tail(hipStream_t s, int dim, int N_int, int N, int n_top, int n_bot, double dt,
-                                double load, const double *rec_cur, double *rec_next)
-{
-    if (N <= N_int) return hipSuccess;
-    k_verlet21_tail<<<nblk(N - N_int, 256), 256, 0, s>>>(dim, N_int, N, n_top, n_bot, dt, load, rec_cur, rec_next);
-    return hipGetLastError();
-}
-
-hipError_t launch_pack_vrec(hipStream_t s, int dim, int N, CD3 v, CD3 b, double *vrec)
-{
-    k_pack_vrec<<<nblk(N, 256), 256, 0, s>>>(dim, N, v, b, vrec);
-    return hipGetLastError();
-}
-
-hipError_t launch_unpack_uv(hipStream_t s, int dim, int N, const double *rec, const double *vrec, D3 u, D3 v)
-{
-    k_unpack_uv<<<nblk(N, 256), 256, 0, s>>>(dim, N, rec, vrec, u, v);
-    return hipGetLastError();
-}
-
 hipError_t launch_verlet21(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double dens,
-                           double load, D3 v, CD3 f, CF3 f32, int use32, CD3 b, D3 u, double *rec)
+                           double load, D3 v, CD3 f, CF3 f32, int use32, CD3 b, D3 u, RecU rec)
 {
     int n = N_int + n_top + n_bot;
     if (n < 1) n = 1;
@@ -601,7 +541,7 @@ hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, in
 }
 
 hipError_t launch_adr_update(hipStream_t s, int dim, int N_int, double mass, double dt, CD3 f, CF3 f32,
-                             int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S, double *rec)
+                             int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo, DevScalars *S, RecU rec)
 {
     if (N_int <= 0) return hipSuccess;
     k_adr_update<<<nblk(N_int, 256), 256, 0, s>>>(dim, N_int, mass, dt, f, f32, use32, b, vh, v, vho, u, fo, S,
@@ -617,7 +557,7 @@ hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done)
 
 hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u, D3 v,
                           pd_cyl cyl, double dt, double dens, double vol, double *partials, DevScalars *S,
-                          double *rec, double *terms)
+                          RecU rec, double *terms)
 {
     k_contact_terms<<<nblk(N, 256), 256, 0, s>>>(N, dmg, x, u, v, cyl, dt, dens, vol, S, terms, rec);
     hipError_t e = hipGetLastError();

@@ -208,7 +208,7 @@ __global__ void k_lps_weights(int N, const int32_t *__restrict__ rp, const doubl
 }
 
 __global__ void k_pmb_coef(int64_t nb, const double *__restrict__ fac, const double *__restrict__ scr,
-                           double vol, double bc, const double *__restrict__ idist, double *kappa,
+                           double vol, double bc, const double *__restrict__ idist, double iq, double *kappa,
                            float2 *bnd)
 {
     // kappa = vol*fac*bc*scr: the left prefix of Force_Bond.cu:190, so
@@ -217,17 +217,17 @@ __global__ void k_pmb_coef(int64_t nb, const double *__restrict__ fac, const dou
     if (k >= nb) return;
     double kap = vol * fac[k] * bc * scr[k];
     kappa[k] = kap;
-    bnd[k] = make_float2((float)idist[k], (float)kap);
+    bnd[k] = make_float2((float)(idist[k] * iq), (float)kap);  // |xi| in units of the record quantum
 }
 
 __global__ void k_lps_coef(int64_t nb, const double *__restrict__ w, const double *__restrict__ fac,
-                           double vol, const double *__restrict__ idist, float2 *bnd)
+                           double vol, const double *__restrict__ idist, double iq, float2 *bnd)
 {
     // omega = w*fac*vol per bond, the bond factor shared by the dilatation
     // and force sums of Force_State.cu:224,274-279.
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
-    bnd[k] = make_float2((float)idist[k], (float)(w[k] * fac[k] * vol));
+    bnd[k] = make_float2((float)(idist[k] * iq), (float)(w[k] * fac[k] * vol));
 }
 
 __global__ void k_inv_to_f32(const double *a, float *b, int n)
@@ -333,17 +333,17 @@ hipError_t launch_lps_weights(hipStream_t s, int N, const int32_t *rp, const dou
 }
 
 hipError_t launch_pmb_coef(hipStream_t s, int64_t nb, const double *fac, const double *scr, double vol,
-                           double bc, const double *idist, double *kappa, float2 *bnd)
+                           double bc, const double *idist, double iq, double *kappa, float2 *bnd)
 {
     if (nb <= 0) return hipSuccess;
-    k_pmb_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, fac, scr, vol, bc, idist, kappa, bnd);
+    k_pmb_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, fac, scr, vol, bc, idist, iq, kappa, bnd);
     return hipGetLastError();
 }
 
 hipError_t launch_lps_coef(hipStream_t s, int64_t nb, const double *w, const double *fac, double vol,
-                           const double *idist, float2 *bnd, int N, const double *m, float *m32)
+                           const double *idist, double iq, float2 *bnd, int N, const double *m, float *m32)
 {
-    if (nb > 0) k_lps_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, w, fac, vol, idist, bnd);
+    if (nb > 0) k_lps_coef<<<nblk(nb, 256), 256, 0, s>>>(nb, w, fac, vol, idist, iq, bnd);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_inv_to_f32<<<nblk(N, 256), 256, 0, s>>>(m, m32, N);

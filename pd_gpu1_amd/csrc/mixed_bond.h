@@ -7,8 +7,6 @@
 
 namespace {
 
-constexpr int kRec = 8;  // doubles per particle record
-
 __device__ __forceinline__ float frsq(float v) { return __builtin_amdgcn_rsqf(v); }
 __device__ __forceinline__ float frcp(float v) { return __builtin_amdgcn_rcpf(v); }
 
@@ -20,8 +18,10 @@ __device__ __forceinline__ int xcd_block()
     return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
 }
 
+// A particle record (pd_internal.h, kRecW words) in registers.
 struct Rec {
-    double x0, x1, x2, u0, u1, u2;
+    int32_t ix, iy, iz;  // quantized reference position (modulo 2^32)
+    float ux, uy, uz;    // displacement, FP32
 };
 
 struct RecQ {
@@ -29,12 +29,23 @@ struct RecQ {
     float q, m;  // theta/m and 1/m
 };
 
-__device__ __forceinline__ RecQ load_rec(const double *__restrict__ rec, int c)
+__device__ __forceinline__ RecQ rec_of(uint4 a, uint4 b)
 {
-    const double2 *p = reinterpret_cast<const double2 *>(rec + (size_t)c * kRec);
-    const double2 a = p[0], b = p[1], d = p[2], e = p[3];
-    const float2 qr = __builtin_bit_cast(float2, b.y);
-    return RecQ{Rec{a.x, a.y, b.x, d.x, d.y, e.x}, qr.x, qr.y};
+    return RecQ{Rec{(int32_t)a.x, (int32_t)a.y, (int32_t)a.z, __uint_as_float(b.x), __uint_as_float(b.y),
+                    __uint_as_float(b.z)},
+                __uint_as_float(a.w), __uint_as_float(b.w)};
+}
+
+__device__ __forceinline__ RecQ load_rec(const uint32_t *__restrict__ rec, int c)
+{
+    const uint4 *p = reinterpret_cast<const uint4 *>(rec + (size_t)c * kRecW);
+    return rec_of(p[0], p[1]);
+}
+
+// Largest displacement component of a record.
+__device__ __forceinline__ float umag(const Rec &r)
+{
+    return fmaxf(fmaxf(fabsf(r.ux), fabsf(r.uy)), fabsf(r.uz));
 }
 
 __device__ __forceinline__ bool bit_of(uint32_t word, int32_t k) { return (word >> (k & 31)) & 1u; }
@@ -56,11 +67,13 @@ __device__ __forceinline__ float group_sum(double a)
     return (float)a;
 }
 
-// Geometry of one bond in FP32 from FP64 differences.  Every fused
+// Geometry of one bond in FP32 from the records, in units of the record
+// quantum q: xi from the difference of the quantized positions (exact as an
+// integer, rounded once to FP32), eta from the FP32 displacements.  Every fused
 // multiply-add is written out (the library is built without contraction), so
 // every inlined copy of a bond body rounds the same way, whichever kernel,
-// lane count or unrolling it sits in.  The FP64 parity stretch below keeps
-// the separate roundings of the oracle.
+// tile, lane count or unrolling it sits in.  The FP64 parity stretch below
+// keeps the separate roundings of the oracle.
 template <int DIM>
 struct BondGeo {
     float dx, dy, dz;   // xi + eta
@@ -72,11 +85,18 @@ struct BondGeo {
     __device__ __forceinline__ float len_sum(float L0) const { return __builtin_fmaf(L2, rL, L0); }
 };
 
+// c - i of two quantized coordinates, in units of q: the difference modulo
+// 2^32 is the exact difference of the unreduced integers (|xi| / q < 2^31).
+__device__ __forceinline__ float qdiff(int32_t i, int32_t c)
+{
+    return (float)(int32_t)((uint32_t)c - (uint32_t)i);
+}
+
 template <int DIM>
 __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
 {
-    const float ex = (float)(rc.x0 - ri.x0), ey = (float)(rc.x1 - ri.x1);
-    const float hx = (float)(rc.u0 - ri.u0), hy = (float)(rc.u1 - ri.u1);
+    const float ex = qdiff(ri.ix, rc.ix), ey = qdiff(ri.iy, rc.iy);
+    const float hx = rc.ux - ri.ux, hy = rc.uy - ri.uy;
     BondGeo<DIM> g;
     g.dx = ex + hx;
     g.dy = ey + hy;
@@ -84,7 +104,7 @@ __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
     g.num = __builtin_fmaf(hy, __builtin_fmaf(2.f, ey, hy), hx * __builtin_fmaf(2.f, ex, hx));
     g.xi2 = __builtin_fmaf(ey, ey, ex * ex);
     if (DIM == 3) {
-        const float ez = (float)(rc.x2 - ri.x2), hz = (float)(rc.u2 - ri.u2);
+        const float ez = qdiff(ri.iz, rc.iz), hz = rc.uz - ri.uz;
         g.dz = ez + hz;
         L2 = __builtin_fmaf(g.dz, g.dz, L2);
         g.num = __builtin_fmaf(hz, __builtin_fmaf(2.f, ez, hz), g.num);
@@ -114,10 +134,22 @@ __device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &
     if (W) coef *= __builtin_amdgcn_exp2f(gc.k2 * xi2);
 }
 
+// FP64 state of one particle, read from the SoA arrays for the FP64 stretch.
+struct Rec64 {
+    double x0, x1, x2, u0, u1, u2;
+};
+
+template <int DIM>
+__device__ __forceinline__ Rec64 rec64(const MixCtx &mc, int i)
+{
+    return Rec64{mc.x.p[0][i], mc.x.p[1][i], DIM == 3 ? mc.x.p[2][i] : 0.0,
+                 mc.u.p[0][i], mc.u.p[1][i], DIM == 3 ? mc.u.p[2][i] : 0.0};
+}
+
 // FP64 stretch with the parity expressions (PMB order (x_c+u_c)-x_i-u_i for
 // every component, LPS-3D order for z when lps3d).
 template <int DIM, bool LPS3Z>
-__device__ __forceinline__ double stretch64(const Rec &ri, const Rec &rc)
+__device__ __forceinline__ double stretch64(const Rec64 &ri, const Rec64 &rc)
 {
     double d0 = rc.x0 + rc.u0 - ri.x0 - ri.u0, d1 = rc.x1 + rc.u1 - ri.x1 - ri.u1;
     double a0 = rc.x0 - ri.x0, a1 = rc.x1 - ri.x1;
@@ -161,35 +193,33 @@ __device__ __forceinline__ float masked(float v, uint32_t mask)
     return __uint_as_float(__float_as_uint(v) & mask);
 }
 
-// Half-width of the band around s0 in which the FP64 stretch decides a break.
-// It is at least near_window*s0 and at least 3e-6, which bounds the error of
-// the FP32 stretch of any bond: with r = |eta|/|xi| and the inputs and the
-// three products of eta.(2 xi + eta) each rounded once, the error is below
-// 2^-24 r (12 + 6 r), and |eta| <= |xi + eta| + |xi| gives r <= 2 + s, so
-// about 1.5e-6 near s0.  Outside the band the FP32 comparison therefore
-// equals the FP64 one for every bond, rotated or not.
-inline float stretch_window(double s0, double near_window)
+// Break decision of bond k between row i and neighbour cl (a particle index,
+// or with MAP a position in the tile's staged list map[sb ...]).  lidx, when
+// given, is the LDS tile kernels' per-bond local index word whose bit 15
+// mirrors the fail bit; it is cleared with the fail bit.  The FP64 state is
+// read only for bonds inside the band.
+template <int DIM, bool LPS3Z, bool MAP>
+__device__ __forceinline__ void check_break(int i, int cl, const int32_t *map, int64_t sb, float s, const Slot &sl,
+                                            int32_t k, float win, const MixCtx &mc, double s0, double ymin,
+                                            uint32_t *fail, uint16_t *lidx = nullptr)
 {
-    const double w = near_window * s0;
-    return (float)(w > 3.0e-6 ? w : 3.0e-6);
-}
-
-// Break decision of one bond.  lidx, when given, is the LDS tile kernels'
-// per-bond local index word whose bit 15 mirrors the fail bit; it is
-// cleared with the fail bit.
-template <int DIM, bool LPS3Z>
-__device__ __forceinline__ void check_break(const Rec &ri, const Rec &rc, float s, const Slot &sl, int32_t k,
-                                            float s0f, float win, double s0, double ymin, uint32_t *fail,
-                                            uint16_t *lidx = nullptr)
-{
-    if (sl.mask && s >= s0f - win) {
-        const bool brk = (s > s0f + win ? true : stretch64<DIM, LPS3Z>(ri, rc) > s0) &&
-                         (DIM == 2 || (ri.x1 >= ymin && rc.x1 >= ymin));
+    if (sl.mask && s >= mc.s0f - win) {
+        const int c = MAP ? map[sb + cl] : cl;
+        bool brk = s > mc.s0f + win;
+        if (!brk) brk = stretch64<DIM, LPS3Z>(rec64<DIM>(mc, i), rec64<DIM>(mc, c)) > s0;
+        if (DIM == 3 && brk) brk = mc.x.p[1][i] >= ymin && mc.x.p[1][c] >= ymin;
         if (brk) {
             bond_break(fail, k);
             if (lidx) atomicAnd(reinterpret_cast<uint32_t *>(lidx) + (k >> 1), ~(0x8000u << (16 * (k & 1))));
         }
     }
+}
+
+// Half-width of the break band of a bond whose ends have displacement
+// components at most mi, mj (mix_window in pd_internal.h).
+__device__ __forceinline__ float bond_window(const MixCtx &mc, float mi, float mj)
+{
+    return __builtin_fmaf(mc.k_u, fmaxf(mi, mj), mc.win0);
 }
 
 } // namespace

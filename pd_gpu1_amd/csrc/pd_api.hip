@@ -97,8 +97,6 @@ void free_particles(pd_handle *h)
     dfree(h->minv32);
     dfree(h->theta32);
     dfree(h->rec);
-    dfree(h->rec2);
-    dfree(h->vrec);
     dfree(h->terms);
     h->rec_u_valid = false;
     dfree(h->order);
@@ -159,17 +157,50 @@ int build_lds_tiles(pd_handle *h)
     return PD_OK;
 }
 
+// 1/q of the mixed records (1 when there are none).
+double rec_iq(const pd_handle *h) { return h->rec_q > 0.0 ? 1.0 / h->rec_q : 1.0; }
+
+// Mixed mode particle records (kRecW words, pd_internal.h).  The quantum q
+// is the power of two 2^(e-29) with delta < 2^e, so a bond's coordinate
+// difference stays below 2^29 q and its integer difference below 2^31 in
+// magnitude; the integers rint(x0 / q) are exact in FP64 for |x0| < 2^52 q.
+// lmin, the shortest reference bond, bounds the break band (mix_window).
+int setup_records(pd_handle *h)
+{
+    const pd_params &P = h->P;
+    hipStream_t s = h->stream;
+    int e2 = 0;
+    (void)std::frexp(P.delta, &e2);
+    h->rec_q = std::ldexp(1.0, e2 - 29);
+    REQUIRE(h->xabs < std::ldexp(h->rec_q, 52), "coordinates too large against the horizon for the mixed records");
+    unsigned long long *dmin = nullptr, bits = ~0ull;
+    LAUNCH(hipMalloc(&dmin, sizeof(bits)));
+    hipError_t e = hipMemcpyAsync(dmin, &bits, sizeof(bits), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_min_bond(s, P.dim, h->N, h->row_ptr, h->col, cd3(h->x), dmin);
+    if (e == hipSuccess) e = hipMemcpyAsync(&bits, dmin, sizeof(bits), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dmin);
+    LAUNCH(e);
+    double m2 = INFINITY;
+    if (bits != ~0ull) std::memcpy(&m2, &bits, sizeof(m2));
+    h->lmin = std::sqrt(m2);
+    REQUIRE(h->Nb == 0 || h->lmin > 8.0 * h->rec_q, "coincident particles: a bond of zero length");
+    LAUNCH(dalloc(&h->rec, (size_t)h->N * kRecW));
+    LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec_q, h->rec));
+    LAUNCH(launch_pack_u(s, nullptr, P.dim, h->N, cd3(h->u), h->rec_q, h->rec));
+    h->rec_u_valid = true;
+    return PD_OK;
+}
+
 int ensure_coef(pd_handle *h)
 {
     if (!h->coef_dirty) return PD_OK;
+    REQUIRE(h->col, "pd_build_neighbors first");
     hipStream_t s = h->stream;
     const pd_params &P = h->P;
     if (P.precision == PD_PREC_MIXED) {
-        LAUNCH(dalloc(&h->rec, (size_t)h->N * 8));
-        LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec));
-        LAUNCH(dalloc(&h->rec2, (size_t)h->N * 8));
-        LAUNCH(launch_pack_x0(s, P.dim, h->N, cd3(h->x), h->rec2));
-        LAUNCH(launch_pack_u(s, nullptr, P.dim, h->N, cd3(h->u), h->rec2));
+        int rc = setup_records(h);
+        if (rc) return rc;
         // Tiled processing order.  LDS-staged tiles: 5 dx cubes in 3D (about
         // 900 staged records, 70 KB of LDS: two 512-thread workgroups per
         // CU), 16 dx squares in 2D.  Gathering kernels: 16 dx tiles in 3D (a
@@ -207,7 +238,7 @@ int ensure_coef(pd_handle *h)
     if (P.model == PD_MODEL_PMB) {
         LAUNCH(dalloc(&h->kappa, h->Nb));
         LAUNCH(dalloc(&h->bnd, h->Nb));
-        LAUNCH(launch_pmb_coef(s, h->Nb, h->fac, h->scr, P.vol, P.bc, h->idist, h->kappa, h->bnd));
+        LAUNCH(launch_pmb_coef(s, h->Nb, h->fac, h->scr, P.vol, P.bc, h->idist, rec_iq(h), h->kappa, h->bnd));
     } else {
         if (!h->have_weights) {
             h->err = "LPS force needs pd_lps_weights first";
@@ -217,24 +248,28 @@ int ensure_coef(pd_handle *h)
             LAUNCH(dalloc(&h->bnd, h->Nb));
             LAUNCH(dalloc(&h->minv32, h->N));
             LAUNCH(dalloc(&h->theta32, h->N));
-            LAUNCH(launch_lps_coef(s, h->Nb, h->w, h->fac, P.vol, h->idist, h->bnd, h->N, h->m, h->minv32));
+            LAUNCH(launch_lps_coef(s, h->Nb, h->w, h->fac, P.vol, h->idist, rec_iq(h), h->bnd, h->N, h->m,
+                                   h->minv32));
         } else {
             LAUNCH(dalloc(&h->e, h->Nb));
         }
     }
     h->geom_ok = false;
     if (P.precision == PD_PREC_MIXED && h->coef_mode != 0 && h->Nb > 0) {
+        // in units of the record quantum q (the kernels measure lengths in q)
+        const double q = h->rec_q;
         GeomCoef gc{};
-        gc.lo = (float)(P.delta - P.dx / 2.0);
-        gc.hi = (float)(P.delta + P.dx / 2.0);
-        gc.inv_dx = (float)(1.0 / P.dx);
+        gc.lo = (float)((P.delta - P.dx / 2.0) / q);
+        gc.hi = (float)((P.delta + P.dx / 2.0) / q);
+        gc.inv_dx = (float)(q / P.dx);
         gc.scale = (float)(P.model == PD_MODEL_PMB ? P.vol * P.bc : P.vol);
-        gc.k2 = P.model == PD_MODEL_LPS ? (float)(-1.4426950408889634 / (P.delta * P.delta)) : 0.f;
+        gc.k2 = P.model == PD_MODEL_LPS ? (float)(-1.4426950408889634 * q * q / (P.delta * P.delta)) : 0.f;
         int *dok = nullptr, ok = 0;
         LAUNCH(hipMalloc(&dok, sizeof(int)));
         const int one = 1;
         hipError_t e = hipMemcpyAsync(dok, &one, sizeof(int), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = launch_geom_check(s, P.dim, h->N, h->row_ptr, h->col, cd3(h->x), h->bnd, gc, dok);
+        if (e == hipSuccess)
+            e = launch_geom_check(s, P.dim, h->N, h->row_ptr, h->col, h->rec, h->bnd, gc, dok);
         if (e == hipSuccess) e = hipMemcpyAsync(&ok, dok, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         (void)hipFree(dok);
@@ -278,18 +313,22 @@ const GeomCoef *gcp(const pd_handle *h) { return h->geom_ok ? &h->gc : nullptr; 
 // Constants of the mixed LPS passes.  With t = w (A theta xi + B e) / m, the
 // pair force (t_i + t_j) vol fac n of Force_State.cu:111-117 (2D: A =
 // (2nu-1)/(nu-1)(2K-32G/9), B = 8G) and Force_State.cu:274-279 (3D: 3K theta
-// w xi/m + 15G/m w (e - theta xi/3), i.e. A = 3K-5G, B = 15G).
-void lps_mixed_consts(const pd_params &P, float *ctheta, float *A, float *B)
+// w xi/m + 15G/m w (e - theta xi/3), i.e. A = 3K-5G, B = 15G).  The kernels
+// measure lengths in units of the record quantum q, so A and B carry a
+// factor q and the dilatation constant a factor q^2 (exact scalings).
+void lps_mixed_consts(const pd_handle *h, float *ctheta, float *A, float *B)
 {
+    const pd_params &P = h->P;
+    const double q = h->rec_q;
     if (P.dim == 2) {
         double c2 = (2.0 * P.pratio - 1.0) / (P.pratio - 1.0);
-        *ctheta = (float)(2.0 * c2);
-        *A = (float)(c2 * (2.0 * P.K - 32.0 * P.G / 9.0));
-        *B = (float)(8.0 * P.G);
+        *ctheta = (float)(2.0 * c2 * q * q);
+        *A = (float)(c2 * (2.0 * P.K - 32.0 * P.G / 9.0) * q);
+        *B = (float)(8.0 * P.G * q);
     } else {
-        *ctheta = 3.0f;
-        *A = (float)(3.0 * P.K - 5.0 * P.G);
-        *B = (float)(15.0 * P.G);
+        *ctheta = (float)(3.0 * q * q);
+        *A = (float)((3.0 * P.K - 5.0 * P.G) * q);
+        *B = (float)(15.0 * P.G * q);
     }
 }
 
@@ -310,7 +349,7 @@ int fp64_group(const pd_handle *h)
 hipError_t sync_rec_u(pd_handle *h)
 {
     if (h->P.precision != PD_PREC_MIXED || !h->rec || h->rec_u_valid) return hipSuccess;
-    hipError_t e = launch_pack_u(h->stream, nullptr, h->P.dim, h->N, cd3(h->u), h->rec);
+    hipError_t e = launch_pack_u(h->stream, nullptr, h->P.dim, h->N, cd3(h->u), h->rec_q, h->rec);
     if (e == hipSuccess) h->rec_u_valid = true;
     return e;
 }
@@ -329,7 +368,23 @@ hipError_t sync_lidx_fail(pd_handle *h)
 // Every change of the fail words outside the LDS kernels goes through here.
 void fail_changed(pd_handle *h) { h->lidx_fail_ok = false; }
 
-double *rec_or_null(const pd_handle *h) { return h->P.precision == PD_PREC_MIXED ? h->rec : nullptr; }
+RecU rec_or_null(const pd_handle *h)
+{
+    return RecU{h->P.precision == PD_PREC_MIXED ? h->rec : nullptr, rec_iq(h)};
+}
+
+// Band and FP64 state of the mixed kernels' break decision (mixed_bond.h).
+MixCtx mix_ctx(const pd_handle *h)
+{
+    const pd_params &P = h->P;
+    MixCtx mc{};
+    mc.s0f = (float)P.s0;
+    mix_window(P.s0, P.near_window, h->rec_q, h->lmin, &mc.win0, &mc.k_u);
+    mc.k_u = (float)((double)mc.k_u * h->rec_q);  // M in units of q (exact)
+    mc.x = cd3(h->x);
+    mc.u = cd3(h->u);
+    return mc;
+}
 
 // Force evaluation (PMB, or LPS theta pass + force pass) on the handle's
 // stream.  stop may be null.
@@ -352,10 +407,10 @@ int force_launch(pd_handle *h, const int *stop)
             if (!h->lds_ok) fail_changed(h);
             if (h->lds_ok)
                 LAUNCH(launch_pmb_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, h->fail,
-                                      h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
+                                      h->rec, h->f32, P.s0, mix_ctx(h), ymin, gcp(h)));
             else
                 LAUNCH(launch_pmb_mixed(s, stop, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col, h->bnd,
-                                        h->fail, h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
+                                        h->fail, h->rec, h->f32, P.s0, mix_ctx(h), ymin, gcp(h)));
         }
     } else {
         // theta for i < N (Force_State.cu:339), force for i < N_int (:359)
@@ -370,21 +425,23 @@ int force_launch(pd_handle *h, const int *stop)
         } else {
             const int g = pick_group(h);
             float ctheta, A, B;
-            lps_mixed_consts(P, &ctheta, &A, &B);
+            lps_mixed_consts(h, &ctheta, &A, &B);
             LAUNCH(sync_rec_u(h));
             LAUNCH(sync_lidx_fail(h));
             if (!h->lds_ok) fail_changed(h);
+            const MixCtx mc = mix_ctx(h);
             if (h->lds_ok) {
                 LAUNCH(launch_lps_theta_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
                                             h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
                 LAUNCH(launch_lps_force_lds(s, stop, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr, h->bnd,
-                                            h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32, gcp(h)));
+                                            h->fail, h->rec, A, B, P.s0, mc, ymin, h->f32, gcp(h)));
             } else {
                 LAUNCH(launch_lps_theta_mixed(s, stop, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd,
-                                              h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
+                                              h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
+                                              h->theta32, gcp(h)));
                 LAUNCH(launch_lps_force_mixed(s, stop, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr,
-                                              h->col, h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin,
-                                              h->f32, gcp(h)));
+                                              h->col, h->bnd, h->fail, h->rec, A, B, P.s0, mc, ymin, h->f32,
+                                              gcp(h)));
             }
         }
     }
@@ -414,32 +471,6 @@ int contact_launch(pd_handle *h)
     const pd_params &P = h->P;
     LAUNCH(launch_contact(h->stream, h->red_blocks, h->red_threads, h->N_int, h->dmg, cd3(h->x), h->u, h->v,
                           h->cyl, P.dt, P.dens, P.vol, h->partials, h->S, rec_or_null(h), h->terms));
-    return PD_OK;
-}
-
-// LDS tiles, mixed precision: one step's force passes over the records in
-// cur, each row finishing with half 2 of this step and half 1 of the next;
-// the new u goes to the records in next (all rows: the LPS force pass
-// visits rows < N_int, k_verlet21_tail the rest).
-int fused_lds_step(pd_handle *h, double *cur, double *next)
-{
-    const pd_params &P = h->P;
-    hipStream_t s = h->stream;
-    const double ymin = P.dim == 3 ? P.break_y_min : -INFINITY;
-    const VerletFuse vf{h->N_int, P.n_top, P.n_bot, P.dt, P.dens, P.strip_velocity, h->vrec, next};
-    if (P.model == PD_MODEL_PMB) {
-        LAUNCH(launch_pmb_lds(s, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, h->fail, cur,
-                              h->f32, P.s0, P.near_window, ymin, gcp(h), &vf));
-    } else {
-        float ctheta, A, B;
-        lps_mixed_consts(P, &ctheta, &A, &B);
-        LAUNCH(launch_lps_theta_lds(s, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd, cur,
-                                    h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
-        LAUNCH(launch_lps_force_lds(s, nullptr, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr, h->bnd,
-                                    h->fail, cur, A, B, P.s0, P.near_window, ymin, h->f32, gcp(h), &vf));
-        LAUNCH(launch_verlet21_tail(s, P.dim, h->N_int, h->N, P.n_top, P.n_bot, P.dt, P.strip_velocity, cur,
-                                    next));
-    }
     return PD_OK;
 }
 
@@ -767,6 +798,7 @@ int pd_set_positions(pd_handle *h, const double *x, const double *y, const doubl
     for (int d = 0; d < 3; d++) dfree(h->react[d]);
     h->N = n;
     h->N_int = n_int;
+    h->xabs = 0.0;
     D3 *sets[] = {&h->x, &h->u, &h->v, &h->a, &h->b, &h->f, &h->vh, &h->vho, &h->fo};
     for (D3 *s : sets) LAUNCH(alloc_set(*s, n, h->stream));
     {
@@ -793,6 +825,7 @@ int pd_set_positions(pd_handle *h, const double *x, const double *y, const doubl
             hi = src[d][i] > hi ? src[d][i] : hi;
         }
         REQUIRE(std::isfinite(lo) && std::isfinite(hi), "non-finite coordinate");
+        h->xabs = std::max(h->xabs, std::max(std::fabs(lo), std::fabs(hi)));
         h->lo[d] = lo;
         double cs = h->P.delta * (1.0 + 1e-9);
         h->ncell[d] = (int)std::floor((hi - lo) / cs) + 1;
@@ -1000,16 +1033,18 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
     double acc[2] = {0.0, 0.0};
     LAUNCH(sync_lidx_fail(h));
     if (!h->lds_ok || P.precision == PD_PREC_FP64) fail_changed(h);
+    const MixCtx mc = P.precision == PD_PREC_MIXED ? mix_ctx(h) : MixCtx{};
     for (int r = 0; r < reps && rc == PD_OK; ++r) {
-        if (P.precision == PD_PREC_MIXED) LAUNCH(launch_pack_u(h->stream, nullptr, P.dim, h->N, cd3(h->u), h->rec));
+        if (P.precision == PD_PREC_MIXED)
+            LAUNCH(launch_pack_u(h->stream, nullptr, P.dim, h->N, cd3(h->u), h->rec_q, h->rec));
         LAUNCH(hipEventRecord(e[0], h->stream));
         if (P.model == PD_MODEL_PMB && P.precision == PD_PREC_MIXED) {
             if (h->lds_ok)
                 LAUNCH(launch_pmb_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
-                                      h->fail, h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
+                                      h->fail, h->rec, h->f32, P.s0, mc, ymin, gcp(h)));
             else
                 LAUNCH(launch_pmb_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h), h->N, h->row_ptr, h->col,
-                                        h->bnd, h->fail, h->rec, h->f32, P.s0, P.near_window, ymin, gcp(h)));
+                                        h->bnd, h->fail, h->rec, h->f32, P.s0, mc, ymin, gcp(h)));
             LAUNCH(hipEventRecord(e[1], h->stream));
             LAUNCH(hipEventRecord(e[2], h->stream));
         } else if (P.model == PD_MODEL_PMB) {
@@ -1028,11 +1063,11 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
         } else {
             const int g = pick_group(h);
             float ctheta, A, B;
-            lps_mixed_consts(P, &ctheta, &A, &B);
+            lps_mixed_consts(h, &ctheta, &A, &B);
             if (h->lds_ok)
                 LAUNCH(launch_lps_theta_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr,
-                                            h->bnd, h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32,
-                                            gcp(h)));
+                                            h->bnd, h->rec, h->minv32, ctheta, P.theta_skip_broken,
+                                            h->theta32, gcp(h)));
             else
                 LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, h->order, P.dim, g, h->N, h->row_ptr, h->col,
                                               h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
@@ -1040,11 +1075,11 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
             LAUNCH(hipEventRecord(e[1], h->stream));
             if (h->lds_ok)
                 LAUNCH(launch_lps_force_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr,
-                                            h->bnd, h->fail, h->rec, A, B, P.s0, P.near_window, ymin, h->f32, gcp(h)));
+                                            h->bnd, h->fail, h->rec, A, B, P.s0, mc, ymin, h->f32, gcp(h)));
             else
                 LAUNCH(launch_lps_force_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h, 1), h->N_int,
-                                              h->row_ptr, h->col, h->bnd, h->fail, h->rec, A, B, P.s0,
-                                              P.near_window, ymin, h->f32, gcp(h)));
+                                              h->row_ptr, h->col, h->bnd, h->fail, h->rec, A, B, P.s0, mc, ymin,
+                                              h->f32, gcp(h)));
             LAUNCH(hipEventRecord(e[2], h->stream));
         }
         LAUNCH(hipEventSynchronize(e[2]));
@@ -1141,17 +1176,6 @@ int pd_set_lds(pd_handle *h, int mode)
     REQUIRE(mode >= -1 && mode <= 1, "mode must be -1, 0 or 1");
     h->lds_mode = mode;
     h->coef_dirty = true;
-    drop_graph(h);
-    return PD_OK;
-}
-
-int pd_set_fused_verlet(pd_handle *h, int mode)
-{
-    // Verlet update inside the LDS force pass of pd_run_verlet: 1 on, 0 or
-    // -1 (default) off.  The results are the same either way.
-    if (!h) return PD_E_INVALID;
-    REQUIRE(mode >= -1 && mode <= 1, "mode must be -1, 0 or 1");
-    h->fused_verlet = mode;
     drop_graph(h);
     return PD_OK;
 }
@@ -1279,34 +1303,6 @@ int pd_run_verlet(pd_handle *h, int nsteps)
             if (rc) return rc;
             for (int s = 0; s < nsteps; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
         }
-    } else if (h->P.precision == PD_PREC_MIXED && h->lds_ok && h->fused_verlet == 1) {
-        // half 1; then steps whose force passes also apply half 2 and the
-        // next half 1, alternating between the two record buffers (a graph
-        // of two steps returns to the first buffer); the last step's force
-        // and half 2.
-        rc = verlet1_launch(h);
-        if (rc) return rc;
-        LAUNCH(dalloc(&h->vrec, (size_t)h->N * 8));
-        LAUNCH(launch_pack_vrec(h->stream, h->P.dim, h->N, cd3(h->v), cd3(h->b), h->vrec));
-        const int fused = nsteps - 1;
-        if (fused >= 2) {
-            rc = capture(h, 5, 1, [&]() {
-                int r = fused_lds_step(h, h->rec, h->rec2);
-                return r ? r : fused_lds_step(h, h->rec2, h->rec);
-            });
-            if (rc) return rc;
-            for (int s = 0; s < fused / 2; ++s) LAUNCH(hipGraphLaunch(h->step_graph, h->stream));
-        }
-        if (fused & 1) {
-            rc = fused_lds_step(h, h->rec, h->rec2);
-            if (rc) return rc;
-            std::swap(h->rec, h->rec2);
-            drop_graph(h);  // captured graphs hold the record pointers
-        }
-        LAUNCH(launch_unpack_uv(h->stream, h->P.dim, h->N, h->rec, h->vrec, h->u, h->v));
-        rc = force_launch(h, nullptr);
-        if (!rc) rc = verlet2_launch(h);
-        if (rc) return rc;
     } else {
         // half 1, then (force, half 2 + half 1 of the next step) from a
         // graph, and the last step's force and half 2

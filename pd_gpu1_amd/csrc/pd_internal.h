@@ -68,25 +68,69 @@ struct DevScalars {
 
 // Bond coefficients recomputed from |xi| in the mixed kernels (see geom_coef
 // in mixed_bond.h): coef = scale * fac(|xi|) [* 2^(k2 |xi|^2)].
-// Verlet half 2 of the current step and half 1 of the next, applied by the
-// LDS force kernels to each row as soon as its force is final (the
-// arithmetic of k_verlet2 followed by k_verlet1).  During a run v and b live
-// in a 64-byte record per particle, vrec = {v0, v1, v2, b0, b1, b2, 0, 0},
-// u is taken from the row's own particle record, and the new u goes to the
-// other record buffer, rec_next, so tiles still staging the current records
-// never see it; the SoA u and v are unpacked once the run ends.
-struct VerletFuse {
-    int N_int, n_top, n_bot;
-    double dt, dens, load;
-    double *vrec;
-    double *rec_next;
-};
-
 struct GeomCoef {
     float lo, hi, inv_dx;  // fac = 1 up to lo, (hi - |xi|)/dx up to hi, 0 beyond
     float scale;           // vol (LPS omega = w fac vol) or vol*bc (PMB kappa, scr = 1)
     float k2;              // LPS: -log2(e)/delta^2, so w = 2^(k2 |xi|^2); PMB: 0
 };
+
+// Mixed mode particle record, 8 words (32 bytes) per particle:
+//   words 0..2  reference position, quantized: I = rint(x0 / q) mod 2^32
+//   word  3     theta/m (float), written by the LPS dilatation pass
+//   words 4..6  displacement u rounded to FP32, in units of q: RN(u) / q
+//   word  7     1/m (float)
+// q is a power of two small against the horizon, so the FP32 bond vector
+// (float)(int32)(I_c - I_i), in units of q, differs from the exact one by at
+// most one unit per component and is the same whichever kernel or tile forms
+// it.  The mixed kernels work in units of q throughout (power-of-two scalings
+// change no rounding).
+constexpr int kRecW = 8;
+
+// Records to refresh with a new u (rec null: none) and 1/q.
+struct RecU {
+    uint32_t *rec;
+    double iq;
+};
+
+// What the mixed kernels need besides their bond streams: the quantum, the
+// critical stretch, the half-width of the band in which a bond's break is
+// decided by its FP64 stretch -- win0 + k_u * M for a bond whose end
+// displacements are at most M per component (FP32), see mix_window below --
+// and the FP64 state that decision reads.
+struct MixCtx {
+    float s0f;         // critical stretch, FP32
+    float win0, k_u;   // band half-width (k_u per unit q of M)
+    CD3 x, u;          // FP64 reference positions and displacements
+};
+
+// Half-width of the band around s0 in which the FP64 stretch decides a break.
+// Its floor is at least near_window*s0 and at least 3e-6, which bounds the
+// FP32 evaluation of the stretch formula: with r = |eta|/|xi| and the inputs
+// and the three products of eta.(2 xi + eta) each rounded once, the error is
+// below 2^-24 r (12 + 6 r), and |eta| <= |xi + eta| + |xi| gives r <= 2 + s,
+// so about 1.5e-6 near s0.  The records perturb the inputs before that
+// rounding: the quantized positions move xi by at most d = sqrt(3) q, the FP32
+// displacements move eta by at most e = sqrt(3) 2^-23 (1 + 2^-23) M, M the
+// larger displacement component of the two ends.  As |xi'+eta'| is within
+// d + e of |xi+eta| and |xi'| within d of |xi|, the stretch of the perturbed
+// bond is within ((2 + s) d + e) / (|xi| - d) of s.  mix_window adds that,
+// with s <= s0 + 0.01 and |xi| >= lmin, the shortest reference bond, as
+// win0 + k_u * M; outside the band the FP32 comparison therefore equals the
+// FP64 one for every bond, rotated or not.
+inline float stretch_window(double s0, double near_window)
+{
+    const double w = near_window * s0;
+    return (float)(w > 3.0e-6 ? w : 3.0e-6);
+}
+
+inline void mix_window(double s0, double near_window, double q, double lmin, float *win0, float *k_u)
+{
+    const double d = 1.7320508075688772 * q;
+    const double den = lmin - d;
+    const double slack = 1.0 + 1.0e-6;  // covers the FP32 rounding of win0 + k_u * M
+    *win0 = (float)(slack * ((double)stretch_window(s0, near_window) + (2.01 + s0) * d / den));
+    *k_u = (float)(slack * 1.7320508075688772 * 0x1p-23 * (1.0 + 0x1p-23) / den);
+}
 
 // Per-tile staged neighbour lists of the LDS kernels (k_force_lds.hip).
 struct LdsTiles {
@@ -115,15 +159,15 @@ struct pd_handle {
     F3 f32{};                 // mixed-mode force
     double *m = nullptr, *theta = nullptr, *dmg = nullptr;
     float *minv32 = nullptr, *theta32 = nullptr;  // mixed LPS: 1/m and dilatation
-    double *rec = nullptr;        // mixed mode: 64-byte particle records (x0 | theta/m, 1/m | u)
-    bool rec_u_valid = false;     // the u half of every record equals u
-    double *rec2 = nullptr;       // second record buffer of the fused LDS Verlet steps (x0 packed)
-    double *vrec = nullptr;       // {v, b} per particle during fused LDS Verlet runs
+    uint32_t *rec = nullptr;      // mixed mode: 32-byte particle records (see MixCtx)
+    bool rec_u_valid = false;     // the u words of every record equal u rounded to FP32
+    double rec_q = 0.0;           // quantum of the reference positions in the records
+    double lmin = 0.0;            // shortest reference bond length
+    double xabs = 0.0;            // largest |coordinate| of the reference positions
     double *terms = nullptr;      // [4][N] per-particle terms of the ADR and contact reductions
     int32_t *order = nullptr;     // mixed mode: tiled processing order (null = index order)
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)
     int lds_mode = -1;            // LDS-staged tiles: -1 where they fit (default), 0 off, 1 required
-    int fused_verlet = -1;        // Verlet update inside the LDS force pass: 1 on, 0 or -1 (default) off
     bool lds_ok = false;          // the LDS tile lists are built and in use
     bool lidx_fail_ok = false;    // bit 15 of every lidx word equals the bond's fail bit
     int lds_block = 512;          // threads per workgroup of the LDS kernels (pass 0)
@@ -138,7 +182,7 @@ struct pd_handle {
     uint32_t *fail = nullptr;
     double *idist = nullptr, *fac = nullptr, *scr = nullptr, *kappa = nullptr;
     double *w = nullptr, *e = nullptr;
-    float2 *bnd = nullptr;    // mixed mode: per bond {|xi|, kappa or omega} in FP32
+    float2 *bnd = nullptr;    // mixed mode: per bond {|xi| / q, kappa or omega} in FP32
     int coef_mode = -1;       // mixed mode: -1 recompute from |xi| where that matches bnd, 0 stream bnd, 1 required
     bool geom_ok = false;     // the kernels recompute the bond coefficients (gc) instead of reading bnd
     GeomCoef gc{};
@@ -218,9 +262,9 @@ hipError_t launch_crack_kw(hipStream_t s, int N_int, const int32_t *rp, const in
 hipError_t launch_lps_weights(hipStream_t s, int N, const int32_t *rp, const double *idist,
                               const double *fac, double delta, double vol, int w_given, double *w, double *m);
 hipError_t launch_pmb_coef(hipStream_t s, int64_t nb, const double *fac, const double *scr, double vol,
-                           double bc, const double *idist, double *kappa, float2 *bnd);
+                           double bc, const double *idist, double iq, double *kappa, float2 *bnd);
 hipError_t launch_lps_coef(hipStream_t s, int64_t nb, const double *w, const double *fac, double vol,
-                           const double *idist, float2 *bnd, int N, const double *m, float *minv32);
+                           const double *idist, double iq, float2 *bnd, int N, const double *m, float *minv32);
 hipError_t launch_copy16(hipStream_t s, const void *a, void *b, int64_t bytes);
 hipError_t launch_damage(hipStream_t s, int N_int, const int32_t *rp, const uint32_t *fail,
                          const double *fac, double vol, double *dmg);
@@ -239,10 +283,24 @@ hipError_t launch_lps_force_fp64(hipStream_t s, const int *stop, int dim, int N_
                                  double pratio, double K, double G, double s0, double ymin,
                                  uint32_t *fail, D3 f);
 
-// k_force_mixed.hip (particle records: 8 doubles per particle, x0 | u | (theta/m, 1/m))
-hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double *rec);
-hipError_t launch_geom_check(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col, CD3 x,
-                             const float2 *bnd, GeomCoef gc, int *ok);
+// k_force_mixed.hip (particle records: kRecW words per particle, see MixCtx)
+hipError_t launch_pack_x0(hipStream_t s, int dim, int N, CD3 x, double q, uint32_t *rec);
+hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, double q, uint32_t *rec);
+hipError_t launch_min_bond(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col, CD3 x,
+                           unsigned long long *out);
+hipError_t launch_geom_check(hipStream_t s, int dim, int N, const int32_t *rp, const int32_t *col,
+                             const uint32_t *rec, const float2 *bnd, GeomCoef gc, int *ok);
+hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
+                            const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
+                            const uint32_t *rec, F3 f, double s0, const MixCtx &mc, double ymin, const GeomCoef *gc);
+hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
+                                  const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
+                                  uint32_t *rec, const float *minv32, float ctheta, int skip_broken,
+                                  float *theta32, const GeomCoef *gc);
+hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
+                                  const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
+                                  const uint32_t *rec, float A, float B, double s0, const MixCtx &mc, double ymin, F3 f,
+                                  const GeomCoef *gc);
 // k_force_lds.hip
 int stage_cap();
 int lds_record_bytes();
@@ -251,35 +309,21 @@ hipError_t launch_stage_tiles(hipStream_t s, int phase, int ntiles, const int32_
                               const int32_t *rp, const int32_t *col, int32_t *count, const int64_t *stage_ptr,
                               int32_t *stage_idx, uint16_t *lidx);
 hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group, const int32_t *order,
-                          const int32_t *rp, const float2 *bnd, uint32_t *fail, const double *rec, F3 f, double s0,
-                          double window, double ymin, const GeomCoef *gc, const VerletFuse *vf = nullptr);
+                          const int32_t *rp, const float2 *bnd, uint32_t *fail, const uint32_t *rec, F3 f, double s0,
+                          const MixCtx &mc, double ymin, const GeomCoef *gc);
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
-                                const int32_t *order, const int32_t *rp, const float2 *bnd, double *rec,
+                                const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *rec,
                                 const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
-                                const double *rec, float A, float B, double s0, double window, double ymin, F3 f,
-                                const GeomCoef *gc, const VerletFuse *vf = nullptr);
-hipError_t launch_pack_u(hipStream_t s, const int *stop, int dim, int N, CD3 u, double *rec);
-hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int n_rows,
-                            const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                            const double *rec, F3 f, double s0, double window, double ymin, const GeomCoef *gc);
-hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
-                                  const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
-                                  double *rec, const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
-hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
-                                  const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                                  const double *rec, float A, float B, double s0, double window, double ymin, F3 f, const GeomCoef *gc);
+                                const uint32_t *rec, float A, float B, double s0, const MixCtx &mc, double ymin, F3 f,
+                                const GeomCoef *gc);
 
 // k_integrate.hip
 hipError_t launch_verlet1(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double load,
-                          D3 v, CD3 a, D3 u, double *cyl_state, double *rec);
-hipError_t launch_verlet21_tail(hipStream_t s, int dim, int N_int, int N, int n_top, int n_bot, double dt,
-                                double load, const double *rec_cur, double *rec_next);
-hipError_t launch_pack_vrec(hipStream_t s, int dim, int N, CD3 v, CD3 b, double *vrec);
-hipError_t launch_unpack_uv(hipStream_t s, int dim, int N, const double *rec, const double *vrec, D3 u, D3 v);
+                          D3 v, CD3 a, D3 u, double *cyl_state, RecU rec);
 hipError_t launch_verlet21(hipStream_t s, int dim, int N_int, int n_top, int n_bot, double dt, double dens,
-                           double load, D3 v, CD3 f, CF3 f32, int use32, CD3 b, D3 u, double *rec);
+                           double load, D3 v, CD3 f, CF3 f32, int use32, CD3 b, D3 u, RecU rec);
 hipError_t launch_verlet2(hipStream_t s, int dim, int N_int, double dt, double dens, D3 v, D3 a, CD3 f,
                           CF3 f32, int use32, CD3 b);
 hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, int N_int, int N,
@@ -287,10 +331,10 @@ hipError_t launch_adr_reduce(hipStream_t s, int blocks, int threads, int dim, in
                              CD3 b, double *partials, DevScalars *S, double tol, double *terms);
 hipError_t launch_adr_update(hipStream_t s, int dim, int N_int, double mass, double dt, CD3 f,
                              CF3 f32, int use32, CD3 b, D3 vh, D3 v, D3 vho, D3 u, D3 fo,
-                             DevScalars *S, double *rec);
+                             DevScalars *S, RecU rec);
 hipError_t launch_contact(hipStream_t s, int blocks, int threads, int N, const double *dmg, CD3 x, D3 u,
                           D3 v, pd_cyl cyl, double dt, double dens, double vol, double *partials,
-                          DevScalars *S, double *rec, double *terms);
+                          DevScalars *S, RecU rec, double *terms);
 hipError_t launch_set_ct(hipStream_t s, DevScalars *S, int ct, int clear_done);
 hipError_t launch_field_sum(hipStream_t s, int blocks, int threads, const double *v, int64_t n,
                             double *partials, DevScalars *S);

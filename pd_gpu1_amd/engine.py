@@ -179,11 +179,6 @@ class Problem:
         """Lanes per row of the FP64 PMB kernel (1, 4, 8, 16, 32; -1 = by dimension)."""
         self._c(lib().pd_set_fp64_group(self._h, group))
 
-    def set_fused_verlet(self, mode):
-        """run_verlet on LDS tiles: 1 = Verlet update inside the force pass,
-        0 or -1 (default) = separate particle passes."""
-        self._c(lib().pd_set_fused_verlet(self._h, mode))
-
     def set_coef_mode(self, mode):
         """Bond coefficients: -1 recompute from |xi| where they match (default), 0 stream, 1 recompute."""
         self._c(lib().pd_set_coef_mode(self._h, mode))

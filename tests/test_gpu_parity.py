@@ -367,39 +367,6 @@ def test_mixed_records_follow_every_displacement_update():
         assert np.array_equal(c.get("F"), f_a)
 
 
-@pytest.mark.parametrize("case", ["pmb2d_strips", "lps2d_strips", "lps3d"])
-def test_fused_verlet_steps_match_staged_steps(case):
-    # run_verlet on LDS tiles applies half 2 and the next half 1 inside the
-    # force pass and alternates two record buffers; step counts of both
-    # parities (with and without the graph, with the buffer swap) must give
-    # exactly the staged half 1 / force / half 2 sequence.
-    if case == "pmb2d_strips":
-        sc = scenarios.c0_plate_pmb(scale=0.6)
-    elif case == "lps2d_strips":
-        sc = scenarios.lps2d_plate(scale=0.5, precision=_lib.PD_PREC_MIXED)
-    else:
-        sc = scenarios.c2_lps_block(nx=24, ny=20, nz=16, precision=_lib.PD_PREC_MIXED)
-    rng = np.random.default_rng(17)
-    u = rng.uniform(-1e-7, 1e-7, size=(sc.p.dim, len(sc.x)))
-    with sc.make(precision=_lib.PD_PREC_MIXED) as a, sc.make(precision=_lib.PD_PREC_MIXED) as b:
-        a.set_fused_verlet(1)
-        a.set("U", u)
-        b.set("U", u)
-        for n in (2, 3, 6, 7, 1, 4):
-            a.run_verlet(n)
-            for _ in range(n):
-                b.verlet_half1()
-                b.force()
-                b.verlet_half2()
-            for field in ("U", "V", "A", "F"):
-                assert np.array_equal(a.get(field), b.get(field)), (n, field)
-            assert np.array_equal(a.fail_mask(), b.fail_mask()), n
-        assert a.lds_info()[0]
-        a.force()  # the records the run leaves behind are the current ones
-        b.force()
-        assert np.array_equal(a.get("F"), b.get("F"))
-
-
 def _force_with(sc, lds, tile, u, precision=_lib.PD_PREC_MIXED):
     with sc.make(precision=precision) as prob:
         prob.set_lds(lds)
