@@ -458,6 +458,7 @@ def pmb_section(steps=50, warmup=5, reps=10, fixed=None):
                         f"{steps} Verlet steps at dt=2e-8",
             "bonds": nb, "value": nb * steps / wall, "unit": "bond-updates/s", "ms_per_step": wall * 1e3 / steps,
             "force_bond_updates_per_s": nb / (ms_force * 1e-3), "groups": shape["groups"],
+            "launch_shape": {"lds_tiles": bool(lds_active), "tile_cells": shape["tile"], "block": shape["block"]},
             "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": b,
                          "ms_per_launch": ms_force, "traffic": traffic, "traffic_source": traffic_src,
