@@ -181,32 +181,37 @@ __device__ __forceinline__ uint32_t load_lidx(const uint16_t *lidx, uint32_t k)
     return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(lidx) + 2u * k);
 }
 
+// A slot past the end of the row (k >= ke) reads the row's first bond kb
+// instead: its local index is then one of this tile's, so the neighbour
+// index needs no select (the mask is zero either way).  kb is in bounds of
+// lidx even for an empty row (lidx has Nb + 1 entries or more), whose slots
+// are never dereferenced.
 template <bool GC>
-__device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t ke, const uint16_t *lidx,
+__device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t kb, int32_t ke, const uint16_t *lidx,
                                             const float2 *__restrict__ bnd, bool need_intact)
 {
     const bool valid = k < ke;
-    const uint32_t kk = valid ? (uint32_t)k : 0u;
+    const uint32_t kk = valid ? (uint32_t)k : (uint32_t)kb;
     const uint32_t v = load_lidx(lidx, kk);
-    const float2 b = GC ? make_float2(1.f, 0.f) : bnd[kk];
+    const float2 b = GC ? make_float2(1.f, 0.f) : bnd[valid ? (uint32_t)k : 0u];
     const bool act = valid && (!need_intact || (v & kIntact));
-    return Slot{valid ? (int)(v & (kIntact - 1)) : 0, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
+    return Slot{(int)(v & (kIntact - 1)), valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
 }
 
-// Slot of bond k from its already loaded local index word v.
+// Slot of bond k from its already loaded local index word v (lidx_at).
 template <bool GC>
-__device__ __forceinline__ Slot make_lslot(uint32_t v, int32_t k, int32_t ke, const float2 *__restrict__ bnd,
-                                           bool need_intact)
+__device__ __forceinline__ Slot make_lslot(uint32_t v, int32_t k, int32_t kb, int32_t ke,
+                                           const float2 *__restrict__ bnd, bool need_intact)
 {
     const bool valid = k < ke;
     const float2 b = GC ? make_float2(1.f, 0.f) : bnd[valid ? (uint32_t)k : 0u];
     const bool act = valid && (!need_intact || (v & kIntact));
-    return Slot{valid ? (int)(v & (kIntact - 1)) : 0, valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
+    return Slot{(int)(v & (kIntact - 1)), valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
 }
 
-__device__ __forceinline__ uint32_t lidx_at(const uint16_t *lidx, int32_t k, int32_t ke)
+__device__ __forceinline__ uint32_t lidx_at(const uint16_t *lidx, int32_t k, int32_t kb, int32_t ke)
 {
-    return load_lidx(lidx, k < ke ? (uint32_t)k : 0u);
+    return load_lidx(lidx, k < ke ? (uint32_t)k : (uint32_t)kb);
 }
 
 // Row walk over LDS records; same lane/bond assignment as the gathering walk.
@@ -215,12 +220,12 @@ __device__ __forceinline__ uint32_t lidx_at(const uint16_t *lidx, int32_t k, int
     {                                                                                            \
         const int32_t kb = rp[i], ke = rp[i + 1];                                                \
         int32_t k = kb + lane;                                                                   \
-        Slot sa = fetch_lslot<GC>(k, ke, lidx, bnd, NEED_INTACT);                                \
-        Slot sb = fetch_lslot<GC>(k + G, ke, lidx, bnd, NEED_INTACT);                            \
+        Slot sa = fetch_lslot<GC>(k, kb, ke, lidx, bnd, NEED_INTACT);                                \
+        Slot sb = fetch_lslot<GC>(k + G, kb, ke, lidx, bnd, NEED_INTACT);                            \
         while (k < ke) {                                                                         \
             const RecQ ra = lds_rec(lds, sa.c), rb = lds_rec(lds, sb.c);                         \
-            const Slot na = fetch_lslot<GC>(k + 2 * G, ke, lidx, bnd, NEED_INTACT);              \
-            const Slot nb = fetch_lslot<GC>(k + 3 * G, ke, lidx, bnd, NEED_INTACT);              \
+            const Slot na = fetch_lslot<GC>(k + 2 * G, kb, ke, lidx, bnd, NEED_INTACT);              \
+            const Slot nb = fetch_lslot<GC>(k + 3 * G, kb, ke, lidx, bnd, NEED_INTACT);              \
             BODY(ra, sa, k);                                                                     \
             BODY(rb, sb, k + G);                                                                 \
             sa = na;                                                                             \
@@ -237,15 +242,15 @@ __device__ __forceinline__ uint32_t lidx_at(const uint16_t *lidx, int32_t k, int
     {                                                                                            \
         const int32_t kb = rp[i], ke = rp[i + 1];                                                \
         int32_t k = kb + lane;                                                                   \
-        Slot sa = fetch_lslot<GC>(k, ke, lidx, bnd, NEED_INTACT);                                \
-        Slot sb = fetch_lslot<GC>(k + G, ke, lidx, bnd, NEED_INTACT);                            \
-        uint32_t wc = lidx_at(lidx, k + 2 * G, ke), wd = lidx_at(lidx, k + 3 * G, ke);           \
+        Slot sa = fetch_lslot<GC>(k, kb, ke, lidx, bnd, NEED_INTACT);                                \
+        Slot sb = fetch_lslot<GC>(k + G, kb, ke, lidx, bnd, NEED_INTACT);                            \
+        uint32_t wc = lidx_at(lidx, k + 2 * G, kb, ke), wd = lidx_at(lidx, k + 3 * G, kb, ke);           \
         while (k < ke) {                                                                         \
             const RecQ ra = lds_rec(lds, sa.c), rb = lds_rec(lds, sb.c);                         \
-            const Slot na = make_lslot<GC>(wc, k + 2 * G, ke, bnd, NEED_INTACT);                 \
-            const Slot nb = make_lslot<GC>(wd, k + 3 * G, ke, bnd, NEED_INTACT);                 \
-            wc = lidx_at(lidx, k + 4 * G, ke);                                                   \
-            wd = lidx_at(lidx, k + 5 * G, ke);                                                   \
+            const Slot na = make_lslot<GC>(wc, k + 2 * G, kb, ke, bnd, NEED_INTACT);                 \
+            const Slot nb = make_lslot<GC>(wd, k + 3 * G, kb, ke, bnd, NEED_INTACT);                 \
+            wc = lidx_at(lidx, k + 4 * G, kb, ke);                                                   \
+            wd = lidx_at(lidx, k + 5 * G, kb, ke);                                                   \
             BODY(ra, sa, k);                                                                     \
             BODY(rb, sb, k + G);                                                                 \
             sa = na;                                                                             \
@@ -302,8 +307,9 @@ __device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int n
     }
         PD_LDS_ROW_WALK_PF2(true, PMB_BODY)
 #undef PMB_BODY
-        const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
-        const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
+        const double fs = GC ? (double)gc.scale : 1.0;  // recomputed coefficients leave out vol*bc
+        const float sx = group_sum<G>(ax, fs), sy = group_sum<G>(ay, fs);
+        const float sz = DIM == 3 ? group_sum<G>(az, fs) : 0.f;
         if (lane == 0) {
             f.p[0][i] = sx;
             f.p[1][i] = sy;
@@ -321,8 +327,8 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
                                                         const int32_t *__restrict__ rp,
                                                         const uint16_t *__restrict__ lidx,
                                                         const float2 *__restrict__ bnd, uint32_t *rec,
-                                                        const float *__restrict__ minv32, float ctheta,
-                                                        int skip_broken, float *theta32, GeomCoef gc)
+                                                        const float *__restrict__ minv32, float ctheta, float A,
+                                                        float B, int skip_broken, float *theta32, GeomCoef gc)
 {
     PD_TILE_PROLOGUE(false)
     const bool need_intact = skip_broken != 0;
@@ -346,11 +352,13 @@ __global__ __launch_bounds__(1024) void k_lps_theta_lds(const int *__restrict__ 
 #undef THETA_BODY
         const float th = group_sum<G>(acc);
         if (lane == 0) {
+            // recomputed coefficients leave out vol: the constants carry it
+            const float cth = GC ? ctheta * gc.scale : ctheta, As = GC ? A * gc.scale : A, Bs = GC ? B * gc.scale : B;
             const float rm = minv32[i];
-            const float theta = ctheta * rm * th;
+            const float theta = cth * rm * th;
             theta32[i] = theta;
-            rec[(size_t)i * kRecW + 3] = __float_as_uint(theta * rm);
-            rec[(size_t)i * kRecW + 7] = __float_as_uint(rm);
+            rec[(size_t)i * kRecW + 3] = __float_as_uint(As * (theta * rm));
+            rec[(size_t)i * kRecW + 7] = __float_as_uint(Bs * rm);
         }
     }
 }
@@ -374,7 +382,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1
     const int *__restrict__ stop, int ntiles, const int32_t *__restrict__ tile_start,
     const int32_t *__restrict__ order, const int64_t *__restrict__ stage_ptr, const int32_t *__restrict__ stage_idx,
     const int32_t *__restrict__ rp, uint16_t *lidx, const float2 *__restrict__ bnd, uint32_t *fail,
-    const uint32_t *__restrict__ rec, float A, float B, double s0, MixCtx mc, double ymin, F3 f, GeomCoef gc)
+    const uint32_t *__restrict__ rec, double s0, MixCtx mc, double ymin, F3 f, GeomCoef gc)
 {
     PD_TILE_PROLOGUE(true)
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
@@ -390,7 +398,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
         const float e = g.num * frcp(g.len_sum(L0));                                             \
         const float s = e * (GC ? rL0 : frcp(L0));                                               \
-        const float tt = masked(cf * __builtin_fmaf(A * L0, qi.q + R.q, B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        const float tt = masked(cf * __builtin_fmaf(L0, qi.q + R.q, e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
@@ -451,7 +459,7 @@ hipError_t pmb_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const in
 
 template <int DIM, int G, bool GC>
 hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
-                       const float2 *bnd, uint32_t *rec, const float *minv32, float ctheta,
+                       const float2 *bnd, uint32_t *rec, const float *minv32, float ctheta, float A, float B,
                        int skip_broken, float *theta32, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
@@ -459,22 +467,22 @@ hipError_t theta_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const 
     if (e != hipSuccess) return e;
     k_lps_theta_lds<DIM, G, GC><<<tiles_grid(T.ntiles), T.block, bytes, s>>>(stop, T.ntiles, T.tile_start, order,
                                                                           T.stage_ptr, T.stage_idx, rp, T.lidx, bnd,
-                                                                          rec, minv32, ctheta, skip_broken,
+                                                                          rec, minv32, ctheta, A, B, skip_broken,
                                                                           theta32, gc);
     return hipGetLastError();
 }
 
 template <int DIM, int G, bool GC>
 hipError_t lpsf_lds_g(hipStream_t s, const int *stop, const LdsTiles &T, const int32_t *order, const int32_t *rp,
-                      const float2 *bnd, uint32_t *fail, const uint32_t *rec, float A, float B, double s0,
-                      const MixCtx &mc, double ymin, F3 f, GeomCoef gc)
+                      const float2 *bnd, uint32_t *fail, const uint32_t *rec, double s0, const MixCtx &mc,
+                      double ymin, F3 f, GeomCoef gc)
 {
     const size_t bytes = (size_t)T.max_staged * lds_record_bytes();
     hipError_t e = lds_attr(k_lps_force_lds<DIM, G, GC>, bytes);
     if (e != hipSuccess) return e;
     k_lps_force_lds<DIM, G, GC><<<tiles_grid(T.ntiles_int), T.block2, bytes, s>>>(
-        stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, A, B, s0, mc,
-        ymin, f, gc);
+        stop, T.ntiles_int, T.tile_start, order, T.stage_ptr, T.stage_idx, rp, T.lidx, bnd, fail, rec, s0, mc, ymin,
+        f, gc);
     return hipGetLastError();
 }
 
@@ -513,27 +521,28 @@ hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int
 
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *rec,
-                                const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc)
+                                const float *minv32, float ctheta, float A, float B, int skip_broken, float *theta32,
+                                const GeomCoef *gc)
 {
     if (T.ntiles <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(theta_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta, skip_broken,
-                        theta32)
+        PD_DISPATCH_LDS(theta_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta, A, B,
+                        skip_broken, theta32)
     } else {
-        PD_DISPATCH_LDS(theta_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta, skip_broken,
-                        theta32)
+        PD_DISPATCH_LDS(theta_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, rec, minv32, ctheta, A, B,
+                        skip_broken, theta32)
     }
 }
 
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
-                                const uint32_t *rec, float A, float B, double s0, const MixCtx &mc, double ymin, F3 f,
+                                const uint32_t *rec, double s0, const MixCtx &mc, double ymin, F3 f,
                                 const GeomCoef *gc)
 {
     if (T.ntiles_int <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_LDS(lpsf_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, mc, ymin, f)
+        PD_DISPATCH_LDS(lpsf_lds_g, 3, group, gc, s, stop, T, order, rp, bnd, fail, rec, s0, mc, ymin, f)
     } else {
-        PD_DISPATCH_LDS(lpsf_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, A, B, s0, mc, ymin, f)
+        PD_DISPATCH_LDS(lpsf_lds_g, 2, group, gc, s, stop, T, order, rp, bnd, fail, rec, s0, mc, ymin, f)
     }
 }

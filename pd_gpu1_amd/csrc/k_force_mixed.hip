@@ -85,8 +85,9 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
     }
     PD_ROW_WALK(true, PMB_BODY)
 #undef PMB_BODY
-    const float sx = group_sum<G>(ax), sy = group_sum<G>(ay);
-    const float sz = DIM == 3 ? group_sum<G>(az) : 0.f;
+    const double fs = GC ? (double)gc.scale : 1.0;  // recomputed coefficients leave out vol*bc
+    const float sx = group_sum<G>(ax, fs), sy = group_sum<G>(ay, fs);
+    const float sz = DIM == 3 ? group_sum<G>(az, fs) : 0.f;
     if (lane == 0) {
         f.p[0][i] = sx;
         f.p[1][i] = sy;
@@ -100,8 +101,8 @@ __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__
                                                          const int32_t *__restrict__ col,
                                                          const float2 *__restrict__ bnd,
                                                          const uint32_t *__restrict__ fail, uint32_t *rec,
-                                                         const float *__restrict__ minv32, float ctheta,
-                                                         int skip_broken, float *theta32, GeomCoef gc)
+                                                         const float *__restrict__ minv32, float ctheta, float A,
+                                                         float B, int skip_broken, float *theta32, GeomCoef gc)
 {
     if (stop && *stop) return;
     const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
@@ -123,11 +124,13 @@ __global__ __launch_bounds__(256) void k_lps_theta_mixed(const int *__restrict__
 #undef THETA_BODY
     const float th = group_sum<G>(acc);
     if (lane == 0) {
+        // recomputed coefficients leave out vol: the constants carry it
+        const float cth = GC ? ctheta * gc.scale : ctheta, As = GC ? A * gc.scale : A, Bs = GC ? B * gc.scale : B;
         const float r = minv32[i];
-        const float theta = ctheta * r * th;
+        const float theta = cth * r * th;
         theta32[i] = theta;
-        rec[(size_t)i * kRecW + 3] = __float_as_uint(theta * r);
-        rec[(size_t)i * kRecW + 7] = __float_as_uint(r);
+        rec[(size_t)i * kRecW + 3] = __float_as_uint(As * (theta * r));
+        rec[(size_t)i * kRecW + 7] = __float_as_uint(Bs * r);
     }
 }
 
@@ -136,8 +139,8 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
                                                          const int32_t *__restrict__ rp,
                                                          const int32_t *__restrict__ col,
                                                          const float2 *__restrict__ bnd, uint32_t *fail,
-                                                         const uint32_t *__restrict__ rec, float A, float B,
-                                                         double s0, MixCtx mc, double ymin, F3 f, GeomCoef gc)
+                                                         const uint32_t *__restrict__ rec, double s0, MixCtx mc,
+                                                         double ymin, F3 f, GeomCoef gc)
 {
     if (stop && *stop) return;
     const int gid = (xcd_block() * 256 + (int)threadIdx.x) / G;
@@ -148,7 +151,8 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
     const Rec &ri = qi.r;
     const float mi = umag(ri);
     double ax = 0.0, ay = 0.0, az = 0.0;
-    // (t_i + t_j) vol fac = omega [A xi (q_i + q_j) + B e (r_i + r_j)]
+    // (t_i + t_j) vol fac = omega [xi (Aq_i + Aq_j) + e (Br_i + Br_j)], Aq = A theta/m and
+    // Br = B/m from the records
 #define LPSF_BODY(R, S, K)                                                                       \
     {                                                                                            \
         const BondGeo<DIM> g = geo<DIM>(ri, R.r);                                          \
@@ -156,7 +160,7 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
         if (GC) geom_coef<true>(g.xi2, gc, L0, cf, rL0);                                         \
         const float e = g.num * frcp(g.len_sum(L0));                                             \
         const float s = e * (GC ? rL0 : frcp(L0));                                               \
-        const float tt = masked(cf * __builtin_fmaf(A * L0, qi.q + R.q, B * e * (qi.m + R.m)) * g.rL, S.mask); \
+        const float tt = masked(cf * __builtin_fmaf(L0, qi.q + R.q, e * (qi.m + R.m)) * g.rL, S.mask); \
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
@@ -248,6 +252,7 @@ __global__ void k_geom_check(int dim, int N, const int32_t *__restrict__ rp, con
         else
             geom_coef<false>(xi2, gc, L0, cf, rL0);
         const float2 b = bnd[k];
+        cf *= gc.scale;
         if (!(fabsf(L0 - b.x) <= 1e-6f * b.x && fabsf(cf - b.y) <= 1e-5f * fabsf(b.y) + 1e-30f)) {
             atomicAnd(ok, 0);
             return;
@@ -274,20 +279,20 @@ hipError_t pmb_g(hipStream_t s, const int *stop, const int32_t *order, int n, co
 template <int DIM, int G, bool GC>
 hipError_t theta_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
                    const float2 *bnd, const uint32_t *fail, uint32_t *rec, const float *minv32, float ctheta,
-                   int skip, float *theta32, GeomCoef gc)
+                   float A, float B, int skip, float *theta32, GeomCoef gc)
 {
     k_lps_theta_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec,
-                                                                        minv32, ctheta, skip, theta32, gc);
+                                                                        minv32, ctheta, A, B, skip, theta32, gc);
     return hipGetLastError();
 }
 
 template <int DIM, int G, bool GC>
 hipError_t lpsf_g(hipStream_t s, const int *stop, const int32_t *order, int n, const int32_t *rp, const int32_t *col,
-                  const float2 *bnd, uint32_t *fail, const uint32_t *rec, float A, float B, double s0,
-                  const MixCtx &mc, double ymin, F3 f, GeomCoef gc)
+                  const float2 *bnd, uint32_t *fail, const uint32_t *rec, double s0, const MixCtx &mc,
+                  double ymin, F3 f, GeomCoef gc)
 {
-    k_lps_force_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, A,
-                                                                        B, s0, mc, ymin, f, gc);
+    k_lps_force_mixed<DIM, G, GC><<<grid8((int64_t)n * G), 256, 0, s>>>(stop, order, n, rp, col, bnd, fail, rec, s0,
+                                                                        mc, ymin, f, gc);
     return hipGetLastError();
 }
 
@@ -355,28 +360,28 @@ hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order
 
 hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
-                                  uint32_t *rec, const float *minv32, float ctheta, int skip_broken,
-                                  float *theta32, const GeomCoef *gc)
+                                  uint32_t *rec, const float *minv32, float ctheta, float A, float B,
+                                  int skip_broken, float *theta32, const GeomCoef *gc)
 {
     if (N <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_G(theta_g, 3, group, gc, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta,
+        PD_DISPATCH_G(theta_g, 3, group, gc, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, A, B,
                       skip_broken, theta32)
     } else {
-        PD_DISPATCH_G(theta_g, 2, group, gc, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta,
+        PD_DISPATCH_G(theta_g, 2, group, gc, s, stop, order, N, rp, col, bnd, fail, rec, minv32, ctheta, A, B,
                       skip_broken, theta32)
     }
 }
 
 hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                                  const uint32_t *rec, float A, float B, double s0, const MixCtx &mc, double ymin, F3 f,
+                                  const uint32_t *rec, double s0, const MixCtx &mc, double ymin, F3 f,
                                   const GeomCoef *gc)
 {
     if (N_int <= 0) return hipSuccess;
     if (dim == 3) {
-        PD_DISPATCH_G(lpsf_g, 3, group, gc, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, mc, ymin, f)
+        PD_DISPATCH_G(lpsf_g, 3, group, gc, s, stop, order, N_int, rp, col, bnd, fail, rec, s0, mc, ymin, f)
     } else {
-        PD_DISPATCH_G(lpsf_g, 2, group, gc, s, stop, order, N_int, rp, col, bnd, fail, rec, A, B, s0, mc, ymin, f)
+        PD_DISPATCH_G(lpsf_g, 2, group, gc, s, stop, order, N_int, rp, col, bnd, fail, rec, s0, mc, ymin, f)
     }
 }

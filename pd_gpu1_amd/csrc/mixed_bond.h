@@ -26,7 +26,7 @@ struct Rec {
 
 struct RecQ {
     Rec r;
-    float q, m;  // theta/m and 1/m
+    float q, m;  // A theta/m and B/m of the LPS passes
 };
 
 __device__ __forceinline__ RecQ rec_of(uint4 a, uint4 b)
@@ -60,11 +60,11 @@ __device__ __forceinline__ void bond_break(uint32_t *fail, int32_t k)
 // tighter than Neumaier's in FP32).  The group sum reduces those FP64
 // partials over xor shuffles and rounds once to FP32.
 template <int G>
-__device__ __forceinline__ float group_sum(double a)
+__device__ __forceinline__ float group_sum(double a, double scale = 1.0)
 {
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    return (float)a;
+    return (float)(a * scale);
 }
 
 // Geometry of one bond in FP32 from the records, in units of the record
@@ -121,16 +121,16 @@ __device__ __forceinline__ BondGeo<DIM> geo(const Rec &ri, const Rec &rc)
 // streamed from HBM (used when every stored coefficient equals this
 // recomputation to FP32 accuracy, which the engine checks once per set-up):
 // the partial-volume factor of Surf_Corr.cu:27-32, and for LPS the influence
-// weight exp(-xi^2/delta^2) of Force_State.cu:317.  coef = scale * fac [* w],
-// the weight when W (the LPS passes; gc.k2 is then nonzero).
+// weight exp(-xi^2/delta^2) of Force_State.cu:317.  coef = fac [* w], the
+// weight when W (the LPS passes; gc.k2 is then nonzero); the constant factor
+// gc.scale of the stored coefficients is carried by each pass's constants.
 template <bool W>
 __device__ __forceinline__ void geom_coef(float xi2, const GeomCoef &gc, float &L0, float &coef, float &rL0)
 {
     rL0 = frsq(xi2);
     L0 = xi2 * rL0;
-    // 1 up to lo = delta - dx/2, (hi - |xi|)/dx up to hi = lo + dx, 0 beyond
-    const float fac = __builtin_amdgcn_fmed3f((gc.hi - L0) * gc.inv_dx, 0.f, 1.f);
-    coef = gc.scale * fac;
+    // 1 up to delta - dx/2, (hi - |xi|)/dx up to hi = delta + dx/2, 0 beyond
+    coef = __builtin_amdgcn_fmed3f(__builtin_fmaf(L0, -gc.inv_dx, gc.hi_dx), 0.f, 1.f);
     if (W) coef *= __builtin_amdgcn_exp2f(gc.k2 * xi2);
 }
 

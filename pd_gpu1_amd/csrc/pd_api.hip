@@ -259,8 +259,7 @@ int ensure_coef(pd_handle *h)
         // in units of the record quantum q (the kernels measure lengths in q)
         const double q = h->rec_q;
         GeomCoef gc{};
-        gc.lo = (float)((P.delta - P.dx / 2.0) / q);
-        gc.hi = (float)((P.delta + P.dx / 2.0) / q);
+        gc.hi_dx = (float)((P.delta + P.dx / 2.0) / P.dx);
         gc.inv_dx = (float)(q / P.dx);
         gc.scale = (float)(P.model == PD_MODEL_PMB ? P.vol * P.bc : P.vol);
         gc.k2 = P.model == PD_MODEL_LPS ? (float)(-1.4426950408889634 * q * q / (P.delta * P.delta)) : 0.f;
@@ -315,7 +314,9 @@ const GeomCoef *gcp(const pd_handle *h) { return h->geom_ok ? &h->gc : nullptr; 
 // (2nu-1)/(nu-1)(2K-32G/9), B = 8G) and Force_State.cu:274-279 (3D: 3K theta
 // w xi/m + 15G/m w (e - theta xi/3), i.e. A = 3K-5G, B = 15G).  The kernels
 // measure lengths in units of the record quantum q, so A and B carry a
-// factor q and the dilatation constant a factor q^2 (exact scalings).
+// factor q and the dilatation constant a factor q^2 (exact scalings).  The
+// dilatation pass stores A theta/m and B/m in each particle record for the
+// force pass, and with recomputed coefficients folds vol into the three.
 void lps_mixed_consts(const pd_handle *h, float *ctheta, float *A, float *B)
 {
     const pd_params &P = h->P;
@@ -432,16 +433,16 @@ int force_launch(pd_handle *h, const int *stop)
             const MixCtx mc = mix_ctx(h);
             if (h->lds_ok) {
                 LAUNCH(launch_lps_theta_lds(s, stop, h->lds, P.dim, lds_group(h), h->order, h->row_ptr, h->bnd,
-                                            h->rec, h->minv32, ctheta, P.theta_skip_broken, h->theta32, gcp(h)));
+                                            h->rec, h->minv32, ctheta, A, B, P.theta_skip_broken, h->theta32,
+                                            gcp(h)));
                 LAUNCH(launch_lps_force_lds(s, stop, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr, h->bnd,
-                                            h->fail, h->rec, A, B, P.s0, mc, ymin, h->f32, gcp(h)));
+                                            h->fail, h->rec, P.s0, mc, ymin, h->f32, gcp(h)));
             } else {
                 LAUNCH(launch_lps_theta_mixed(s, stop, h->order, P.dim, g, h->N, h->row_ptr, h->col, h->bnd,
-                                              h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
+                                              h->fail, h->rec, h->minv32, ctheta, A, B, P.theta_skip_broken,
                                               h->theta32, gcp(h)));
                 LAUNCH(launch_lps_force_mixed(s, stop, h->order, P.dim, pick_group(h, 1), h->N_int, h->row_ptr,
-                                              h->col, h->bnd, h->fail, h->rec, A, B, P.s0, mc, ymin, h->f32,
-                                              gcp(h)));
+                                              h->col, h->bnd, h->fail, h->rec, P.s0, mc, ymin, h->f32, gcp(h)));
             }
         }
     }
@@ -1066,20 +1067,20 @@ int pd_time_force_passes(pd_handle *h, int reps, double *ms_out)
             lps_mixed_consts(h, &ctheta, &A, &B);
             if (h->lds_ok)
                 LAUNCH(launch_lps_theta_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h), h->order, h->row_ptr,
-                                            h->bnd, h->rec, h->minv32, ctheta, P.theta_skip_broken,
+                                            h->bnd, h->rec, h->minv32, ctheta, A, B, P.theta_skip_broken,
                                             h->theta32, gcp(h)));
             else
                 LAUNCH(launch_lps_theta_mixed(h->stream, nullptr, h->order, P.dim, g, h->N, h->row_ptr, h->col,
-                                              h->bnd, h->fail, h->rec, h->minv32, ctheta, P.theta_skip_broken,
-                                              h->theta32, gcp(h)));
+                                              h->bnd, h->fail, h->rec, h->minv32, ctheta, A, B,
+                                              P.theta_skip_broken, h->theta32, gcp(h)));
             LAUNCH(hipEventRecord(e[1], h->stream));
             if (h->lds_ok)
                 LAUNCH(launch_lps_force_lds(h->stream, nullptr, h->lds, P.dim, lds_group(h, 1), h->order, h->row_ptr,
-                                            h->bnd, h->fail, h->rec, A, B, P.s0, mc, ymin, h->f32, gcp(h)));
+                                            h->bnd, h->fail, h->rec, P.s0, mc, ymin, h->f32, gcp(h)));
             else
                 LAUNCH(launch_lps_force_mixed(h->stream, nullptr, h->order, P.dim, pick_group(h, 1), h->N_int,
-                                              h->row_ptr, h->col, h->bnd, h->fail, h->rec, A, B, P.s0, mc, ymin,
-                                              h->f32, gcp(h)));
+                                              h->row_ptr, h->col, h->bnd, h->fail, h->rec, P.s0, mc, ymin, h->f32,
+                                              gcp(h)));
             LAUNCH(hipEventRecord(e[2], h->stream));
         }
         LAUNCH(hipEventSynchronize(e[2]));

@@ -69,16 +69,17 @@ struct DevScalars {
 // Bond coefficients recomputed from |xi| in the mixed kernels (see geom_coef
 // in mixed_bond.h): coef = scale * fac(|xi|) [* 2^(k2 |xi|^2)].
 struct GeomCoef {
-    float lo, hi, inv_dx;  // fac = 1 up to lo, (hi - |xi|)/dx up to hi, 0 beyond
-    float scale;           // vol (LPS omega = w fac vol) or vol*bc (PMB kappa, scr = 1)
+    float hi_dx, inv_dx;   // fac = clamp(hi/dx - |xi|/dx, 0, 1), hi = delta + dx/2
+    float scale;           // vol (LPS omega = w fac vol) or vol*bc (PMB kappa, scr = 1), folded
+                           // into each pass's constants rather than applied per bond
     float k2;              // LPS: -log2(e)/delta^2, so w = 2^(k2 |xi|^2); PMB: 0
 };
 
 // Mixed mode particle record, 8 words (32 bytes) per particle:
 //   words 0..2  reference position, quantized: I = rint(x0 / q) mod 2^32
-//   word  3     theta/m (float), written by the LPS dilatation pass
+//   word  3     A theta/m (float), written by the LPS dilatation pass
 //   words 4..6  displacement u rounded to FP32, in units of q: RN(u) / q
-//   word  7     1/m (float)
+//   word  7     B/m (float), likewise (A, B of the LPS force, see lps_mixed_consts)
 // q is a power of two small against the horizon, so the FP32 bond vector
 // (float)(int32)(I_c - I_i), in units of q, differs from the exact one by at
 // most one unit per component and is the same whichever kernel or tile forms
@@ -295,11 +296,11 @@ hipError_t launch_pmb_mixed(hipStream_t s, const int *stop, const int32_t *order
                             const uint32_t *rec, F3 f, double s0, const MixCtx &mc, double ymin, const GeomCoef *gc);
 hipError_t launch_lps_theta_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, const uint32_t *fail,
-                                  uint32_t *rec, const float *minv32, float ctheta, int skip_broken,
-                                  float *theta32, const GeomCoef *gc);
+                                  uint32_t *rec, const float *minv32, float ctheta, float A, float B,
+                                  int skip_broken, float *theta32, const GeomCoef *gc);
 hipError_t launch_lps_force_mixed(hipStream_t s, const int *stop, const int32_t *order, int dim, int group, int N_int,
                                   const int32_t *rp, const int32_t *col, const float2 *bnd, uint32_t *fail,
-                                  const uint32_t *rec, float A, float B, double s0, const MixCtx &mc, double ymin, F3 f,
+                                  const uint32_t *rec, double s0, const MixCtx &mc, double ymin, F3 f,
                                   const GeomCoef *gc);
 // k_force_lds.hip
 int stage_cap();
@@ -313,10 +314,11 @@ hipError_t launch_pmb_lds(hipStream_t s, const int *stop, const LdsTiles &T, int
                           const MixCtx &mc, double ymin, const GeomCoef *gc);
 hipError_t launch_lps_theta_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *rec,
-                                const float *minv32, float ctheta, int skip_broken, float *theta32, const GeomCoef *gc);
+                                const float *minv32, float ctheta, float A, float B, int skip_broken, float *theta32,
+                                const GeomCoef *gc);
 hipError_t launch_lps_force_lds(hipStream_t s, const int *stop, const LdsTiles &T, int dim, int group,
                                 const int32_t *order, const int32_t *rp, const float2 *bnd, uint32_t *fail,
-                                const uint32_t *rec, float A, float B, double s0, const MixCtx &mc, double ymin, F3 f,
+                                const uint32_t *rec, double s0, const MixCtx &mc, double ymin, F3 f,
                                 const GeomCoef *gc);
 
 // k_integrate.hip
