@@ -291,7 +291,7 @@ __device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int n
     for (int r = tile_start[t] + (int)threadIdx.x / G; r < r1; r += (int)blockDim.x / G) {
         const int i = order[r];
         const Rec ri = load_rec(rec, i).r;
-        const float win = bond_window(mc, umag(ri), ut);
+        const float ew = bond_window(mc, umag(ri), ut);
         double ax = 0.0, ay = 0.0, az = 0.0;
 #define PMB_BODY(R, S, K)                                                                        \
     {                                                                                            \
@@ -303,7 +303,8 @@ __device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int n
         ax += (double)(tb * g.dx);                                                               \
         ay += (double)(tb * g.dy);                                                               \
         if (DIM == 3) az += (double)(tb * g.dz);                                                 \
-        check_break<DIM, false, true>(i, S.c, stage_idx, sbase, s, S, K, win, mc, s0, ymin, fail, lidx); \
+        check_break<DIM, false, true>(i, S.c, stage_idx, sbase, s, S, K, ew, ri, R.r, g.xi2, mc, s0, ymin, fail, \
+                                      lidx);                                                     \
     }
         PD_LDS_ROW_WALK_PF2(true, PMB_BODY)
 #undef PMB_BODY
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1
         const int i = order[r];
         const RecQ qi = load_rec(rec, i);
         const Rec &ri = qi.r;
-        const float win = bond_window(mc, umag(ri), ut);
+        const float ew = bond_window(mc, umag(ri), ut);
         double ax = 0.0, ay = 0.0, az = 0.0;
 #define LPSF_BODY(R, S, K)                                                                       \
     {                                                                                            \
@@ -402,7 +403,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(GC ? 6 : 1
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
-        check_break<DIM, true, true>(i, S.c, stage_idx, sbase, s, S, K, win, mc, s0, ymin, fail, lidx); \
+        check_break<DIM, true, true>(i, S.c, stage_idx, sbase, s, S, K, ew, ri, R.r, g.xi2, mc, s0, ymin, fail, \
+                                     lidx);                                                      \
     }
         PD_LDS_ROW_WALK(true, LPSF_BODY)
 #undef LPSF_BODY

@@ -80,8 +80,8 @@ __global__ __launch_bounds__(256) void k_pmb_mixed(const int *__restrict__ stop,
         ax += (double)(t * g.dx);                                                                \
         ay += (double)(t * g.dy);                                                                \
         if (DIM == 3) az += (double)(t * g.dz);                                                  \
-        check_break<DIM, false, false>(i, S.c, nullptr, 0, s, S, K, bond_window(mc, mi, umag(R.r)), mc, s0, ymin, \
-                                       fail);                                                    \
+        check_break<DIM, false, false>(i, S.c, nullptr, 0, s, S, K, bond_window(mc, mi, umag(R.r)), ri, R.r, \
+                                       g.xi2, mc, s0, ymin, fail);                                                    \
     }
     PD_ROW_WALK(true, PMB_BODY)
 #undef PMB_BODY
@@ -164,8 +164,8 @@ __global__ __launch_bounds__(256) void k_lps_force_mixed(const int *__restrict__
         ax += (double)(tt * g.dx);                                                               \
         ay += (double)(tt * g.dy);                                                               \
         if (DIM == 3) az += (double)(tt * g.dz);                                                 \
-        check_break<DIM, true, false>(i, S.c, nullptr, 0, s, S, K, bond_window(mc, mi, umag(R.r)), mc, s0, ymin, \
-                                      fail);                                                     \
+        check_break<DIM, true, false>(i, S.c, nullptr, 0, s, S, K, bond_window(mc, mi, umag(R.r)), ri, R.r, \
+                                      g.xi2, mc, s0, ymin, fail);                                                     \
     }
     PD_ROW_WALK(true, LPSF_BODY)
 #undef LPSF_BODY

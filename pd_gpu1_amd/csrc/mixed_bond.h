@@ -193,21 +193,34 @@ __device__ __forceinline__ float masked(float v, uint32_t mask)
     return __uint_as_float(__float_as_uint(v) & mask);
 }
 
-// Break decision of bond k between row i and neighbour cl (a particle index,
-// or with MAP a position in the tile's staged list map[sb ...]).  lidx, when
-// given, is the LDS tile kernels' per-bond local index word whose bit 15
+// Break decision of bond k between row i (record ri) and neighbour cl (a
+// particle index, or with MAP a position in the tile's staged list
+// map[sb ...]; record rc), s its FP32 stretch, xi2 its |xi|^2, ew = wpos +
+// k_u M the records' part of the band (mix_window in pd_internal.h).  lidx,
+// when given, is the LDS tile kernels' per-bond local index word whose bit 15
 // mirrors the fail bit; it is cleared with the fail bit.  The FP64 state is
-// read only for bonds inside the band.
+// read only for bonds inside the inner band.
 template <int DIM, bool LPS3Z, bool MAP>
 __device__ __forceinline__ void check_break(int i, int cl, const int32_t *map, int64_t sb, float s, const Slot &sl,
-                                            int32_t k, float win, const MixCtx &mc, double s0, double ymin,
-                                            uint32_t *fail, uint16_t *lidx = nullptr)
+                                            int32_t k, float ew, const Rec &ri, const Rec &rc, float xi2,
+                                            const MixCtx &mc, double s0, double ymin, uint32_t *fail,
+                                            uint16_t *lidx = nullptr)
 {
-    if (sl.mask && s >= mc.s0f - win) {
-        const int c = MAP ? map[sb + cl] : cl;
-        bool brk = s > mc.s0f + win;
-        if (!brk) brk = stretch64<DIM, LPS3Z>(rec64<DIM>(mc, i), rec64<DIM>(mc, c)) > s0;
-        if (DIM == 3 && brk) brk = mc.x.p[1][i] >= ymin && mc.x.p[1][c] >= ymin;
+    const float wo = mc.wfloor + ew;
+    if (sl.mask && s >= mc.s0f - wo) {
+        bool brk = s > mc.s0f + wo;  // beyond the outer band: broken, no more work
+        if (!brk) {
+            const float hx = rc.ux - ri.ux, hy = rc.uy - ri.uy, hz = rc.uz - ri.uz;
+            const float r = __builtin_sqrtf(__builtin_fmaf(hz, hz, __builtin_fmaf(hy, hy, hx * hx)) * frcp(xi2));
+            const float wt = fmaxf(mc.wn, 0x1p-24f * r * __builtin_fmaf(16.f, r, 32.f)) + ew;
+            if (s < mc.s0f - wt) return;
+            brk = s > mc.s0f + wt;
+            if (!brk) brk = stretch64<DIM, LPS3Z>(rec64<DIM>(mc, i), rec64<DIM>(mc, MAP ? map[sb + cl] : cl)) > s0;
+        }
+        if (DIM == 3 && brk && ymin > -INFINITY) {
+            const int c = MAP ? map[sb + cl] : cl;
+            brk = mc.x.p[1][i] >= ymin && mc.x.p[1][c] >= ymin;
+        }
         if (brk) {
             bond_break(fail, k);
             if (lidx) atomicAnd(reinterpret_cast<uint32_t *>(lidx) + (k >> 1), ~(0x8000u << (16 * (k & 1))));
@@ -215,11 +228,11 @@ __device__ __forceinline__ void check_break(int i, int cl, const int32_t *map, i
     }
 }
 
-// Half-width of the break band of a bond whose ends have displacement
-// components at most mi, mj (mix_window in pd_internal.h).
+// The records' part of the break band of a bond whose ends have
+// displacement components at most mi, mj.
 __device__ __forceinline__ float bond_window(const MixCtx &mc, float mi, float mj)
 {
-    return __builtin_fmaf(mc.k_u, fmaxf(mi, mj), mc.win0);
+    return __builtin_fmaf(mc.k_u, fmaxf(mi, mj), mc.wpos);
 }
 
 } // namespace

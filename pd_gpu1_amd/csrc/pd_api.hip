@@ -380,7 +380,7 @@ MixCtx mix_ctx(const pd_handle *h)
     const pd_params &P = h->P;
     MixCtx mc{};
     mc.s0f = (float)P.s0;
-    mix_window(P.s0, P.near_window, h->rec_q, h->lmin, &mc.win0, &mc.k_u);
+    mix_window(P.s0, P.near_window, h->rec_q, h->lmin, &mc);
     mc.k_u = (float)((double)mc.k_u * h->rec_q);  // M in units of q (exact)
     mc.x = cd3(h->x);
     mc.u = cd3(h->u);

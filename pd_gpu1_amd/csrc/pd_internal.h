@@ -95,12 +95,13 @@ struct RecU {
 
 // What the mixed kernels need besides their bond streams: the quantum, the
 // critical stretch, the half-width of the band in which a bond's break is
-// decided by its FP64 stretch -- win0 + k_u * M for a bond whose end
-// displacements are at most M per component (FP32), see mix_window below --
-// and the FP64 state that decision reads.
+// decided by its FP64 stretch (mix_window below; M bounds the displacement
+// components of the bond's two ends) and the FP64 state that decision reads.
 struct MixCtx {
     float s0f;         // critical stretch, FP32
-    float win0, k_u;   // band half-width (k_u per unit q of M)
+    float wfloor;      // floor of the outer band, max(near_window s0, 3e-6)
+    float wn;          // near_window s0: the least inner band
+    float wpos, k_u;   // what the records add: wpos + k_u M (k_u per unit q of M)
     CD3 x, u;          // FP64 reference positions and displacements
 };
 
@@ -114,23 +115,31 @@ struct MixCtx {
 // displacements move eta by at most e = sqrt(3) 2^-23 (1 + 2^-23) M, M the
 // larger displacement component of the two ends.  As |xi'+eta'| is within
 // d + e of |xi+eta| and |xi'| within d of |xi|, the stretch of the perturbed
-// bond is within ((2 + s) d + e) / (|xi| - d) of s.  mix_window adds that,
+// bond is within ((2 + s) d + e) / (|xi| - d) of s.  mix_window gives that,
 // with s <= s0 + 0.01 and |xi| >= lmin, the shortest reference bond, as
-// win0 + k_u * M; outside the band the FP32 comparison therefore equals the
-// FP64 one for every bond, rotated or not.
+// wpos + k_u * M.  Outer band: wfloor + wpos + k_u * M, the same for every
+// bond of a row (or every bond), a cheap test on the FP32 stretch; outside it
+// the FP32 comparison equals the FP64 one for every bond, rotated or not.
+// Inner band, for the bonds inside the outer one: the evaluation bound with
+// the bond's own r, 2^-24 r (32 + 16 r) (the 12 + 6 r above with a margin for
+// the approximate reciprocals and for r itself in FP32), but at least
+// near_window * s0, plus wpos + k_u * M.  Only bonds inside the inner band
+// read the FP64 state; every break decision equals the FP64 one.
 inline float stretch_window(double s0, double near_window)
 {
     const double w = near_window * s0;
     return (float)(w > 3.0e-6 ? w : 3.0e-6);
 }
 
-inline void mix_window(double s0, double near_window, double q, double lmin, float *win0, float *k_u)
+inline void mix_window(double s0, double near_window, double q, double lmin, MixCtx *mc)
 {
     const double d = 1.7320508075688772 * q;
     const double den = lmin - d;
-    const double slack = 1.0 + 1.0e-6;  // covers the FP32 rounding of win0 + k_u * M
-    *win0 = (float)(slack * ((double)stretch_window(s0, near_window) + (2.01 + s0) * d / den));
-    *k_u = (float)(slack * 1.7320508075688772 * 0x1p-23 * (1.0 + 0x1p-23) / den);
+    const double slack = 1.0 + 1.0e-6;  // covers the FP32 rounding of the band sums
+    mc->wfloor = (float)(slack * (double)stretch_window(s0, near_window));
+    mc->wn = (float)(slack * near_window * s0);
+    mc->wpos = (float)(slack * (2.01 + s0) * d / den);
+    mc->k_u = (float)(slack * 1.7320508075688772 * 0x1p-23 * (1.0 + 0x1p-23) / den);
 }
 
 // Per-tile staged neighbour lists of the LDS kernels (k_force_lds.hip).
