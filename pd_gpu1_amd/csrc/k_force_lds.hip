@@ -198,22 +198,6 @@ __device__ __forceinline__ Slot fetch_lslot(int32_t k, int32_t kb, int32_t ke, c
     return Slot{(int)(v & (kIntact - 1)), valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
 }
 
-// Slot of bond k from its already loaded local index word v (lidx_at).
-template <bool GC>
-__device__ __forceinline__ Slot make_lslot(uint32_t v, int32_t k, int32_t kb, int32_t ke,
-                                           const float2 *__restrict__ bnd, bool need_intact)
-{
-    const bool valid = k < ke;
-    const float2 b = GC ? make_float2(1.f, 0.f) : bnd[valid ? (uint32_t)k : 0u];
-    const bool act = valid && (!need_intact || (v & kIntact));
-    return Slot{(int)(v & (kIntact - 1)), valid ? b.x : 1.f, b.y, act ? 0xffffffffu : 0u};
-}
-
-__device__ __forceinline__ uint32_t lidx_at(const uint16_t *lidx, int32_t k, int32_t kb, int32_t ke)
-{
-    return load_lidx(lidx, k < ke ? (uint32_t)k : (uint32_t)kb);
-}
-
 // Row walk over LDS records; same lane/bond assignment as the gathering walk.
 // The slots are fetched one bond pair ahead of their use.
 #define PD_LDS_ROW_WALK(NEED_INTACT, BODY)                                                       \
@@ -226,31 +210,6 @@ __device__ __forceinline__ uint32_t lidx_at(const uint16_t *lidx, int32_t k, int
             const RecQ ra = lds_rec(lds, sa.c), rb = lds_rec(lds, sb.c);                         \
             const Slot na = fetch_lslot<GC>(k + 2 * G, kb, ke, lidx, bnd, NEED_INTACT);              \
             const Slot nb = fetch_lslot<GC>(k + 3 * G, kb, ke, lidx, bnd, NEED_INTACT);              \
-            BODY(ra, sa, k);                                                                     \
-            BODY(rb, sb, k + G);                                                                 \
-            sa = na;                                                                             \
-            sb = nb;                                                                             \
-            k += 2 * G;                                                                          \
-        }                                                                                        \
-    }
-
-// The same walk with the index words loaded two bond pairs ahead of their use
-// (they stream from HBM) and the slots one pair ahead: 3% faster for the PMB
-// pass on the 4M-particle block (1.08 -> 1.05 ms), while the LPS passes,
-// whose extra registers cost the force pass a spill, measured 0 and +2%.
-#define PD_LDS_ROW_WALK_PF2(NEED_INTACT, BODY)                                                   \
-    {                                                                                            \
-        const int32_t kb = rp[i], ke = rp[i + 1];                                                \
-        int32_t k = kb + lane;                                                                   \
-        Slot sa = fetch_lslot<GC>(k, kb, ke, lidx, bnd, NEED_INTACT);                                \
-        Slot sb = fetch_lslot<GC>(k + G, kb, ke, lidx, bnd, NEED_INTACT);                            \
-        uint32_t wc = lidx_at(lidx, k + 2 * G, kb, ke), wd = lidx_at(lidx, k + 3 * G, kb, ke);           \
-        while (k < ke) {                                                                         \
-            const RecQ ra = lds_rec(lds, sa.c), rb = lds_rec(lds, sb.c);                         \
-            const Slot na = make_lslot<GC>(wc, k + 2 * G, kb, ke, bnd, NEED_INTACT);                 \
-            const Slot nb = make_lslot<GC>(wd, k + 3 * G, kb, ke, bnd, NEED_INTACT);                 \
-            wc = lidx_at(lidx, k + 4 * G, kb, ke);                                                   \
-            wd = lidx_at(lidx, k + 5 * G, kb, ke);                                                   \
             BODY(ra, sa, k);                                                                     \
             BODY(rb, sb, k + G);                                                                 \
             sa = na;                                                                             \
@@ -306,7 +265,7 @@ __device__ __forceinline__ void pmb_lds_body(const int *__restrict__ stop, int n
         check_break<DIM, false, true>(i, S.c, stage_idx, sbase, s, S, K, ew, ri, R.r, g.xi2, mc, s0, ymin, fail, \
                                       lidx);                                                     \
     }
-        PD_LDS_ROW_WALK_PF2(true, PMB_BODY)
+        PD_LDS_ROW_WALK(true, PMB_BODY)
 #undef PMB_BODY
         const double fs = GC ? (double)gc.scale : 1.0;  // recomputed coefficients leave out vol*bc
         const float sx = group_sum<G>(ax, fs), sy = group_sum<G>(ay, fs);
