@@ -7,6 +7,7 @@
 // arithmetic of Force_State.cu:218-224,263-281.  The mixed-precision
 // throughput kernels are in k_force_mixed.hip.
 #include "pd_internal.h"
+#include "lanes.h"
 
 namespace {
 
@@ -123,13 +124,27 @@ __global__ __launch_bounds__(256) void k_pmb_fp64_group(const int *__restrict__ 
             }
         }
         const int cnt = ke - k0 < G ? ke - k0 : G;
-        for (int l = 0; l < cnt; ++l) {
-            const double sx = __shfl(tx, l, G), sy = __shfl(ty, l, G);
-            const double sz = DIM == 3 ? __shfl(tz, l, G) : 0.0;
-            if (__shfl(ok, l, G)) {
-                fx += sx;
-                fy += sy;
-                if (DIM == 3) fz += sz;
+        if constexpr (G == 4 || G == 8) {
+            // the terms broadcast over DPP (lanes.h) rather than __shfl
+            lanes::Unroll<G>::run([&](auto lc) {
+                constexpr int L = decltype(lc)::value;
+                const double sx = lanes::bcast<G, L>(tx), sy = lanes::bcast<G, L>(ty);
+                const double sz = DIM == 3 ? lanes::bcast<G, L>(tz) : 0.0;
+                if (L < cnt && lanes::bcast<G, L>(ok)) {
+                    fx += sx;
+                    fy += sy;
+                    if (DIM == 3) fz += sz;
+                }
+            });
+        } else {
+            for (int l = 0; l < cnt; ++l) {
+                const double sx = __shfl(tx, l, G), sy = __shfl(ty, l, G);
+                const double sz = DIM == 3 ? __shfl(tz, l, G) : 0.0;
+                if (__shfl(ok, l, G)) {
+                    fx += sx;
+                    fy += sy;
+                    if (DIM == 3) fz += sz;
+                }
             }
         }
     }
