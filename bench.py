@@ -629,9 +629,12 @@ def main():
         "traffic": traffic,
         "traffic_source": traffic_src,
         # what the profile shows limits the kernel: it moves fewer bytes than
-        # the algorithmic count (LDS-staged records, 2-byte index words), and
-        # its VALU and LDS pipes are the busy ones
-        "limiter": "valu+lds (rocprof counters; HBM traffic below the algorithmic bytes)",
+        # the algorithmic count (LDS-staged 32-byte records, 2-byte index
+        # words), and VALU issue is near every cycle (SQ_INSTS_VALU against
+        # 256 CUs x kernel cycles: 0.97 force, 0.96 dilatation); the
+        # dilatation pass's frac passes 1 because its algorithmic count is
+        # what a streaming pass would read
+        "limiter": "valu (rocprof: VALU issue 0.97 of cycles; HBM traffic below the algorithmic bytes)",
         "measured_frac": (round(traffic / (ms_force * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
         "bytes_per_launch": b_force,
         "ms_per_launch": ms_force,
