@@ -147,7 +147,7 @@ __global__ void k_verlet21(int N_int, int n_top, int n_bot, double dt, double de
         double fd[DIM], bd[DIM], vd[DIM];
         for (int d = 0; d < DIM; ++d) {
             fd[d] = F32 ? (double)f32.p[d][i] : f.p[d][i];
-            bd[d] = b.p[d][i];
+            bd[d] = b.p[0] ? b.p[d][i] : 0.0;  // null: the body force is zero
             vd[d] = v.p[d][i];
         }
         for (int d = 0; d < DIM; ++d) {
@@ -174,7 +174,7 @@ __global__ void k_verlet2(int N_int, double dt, double dens, D3 v, D3 a, CD3 f, 
     double fd[DIM], bd[DIM], vd[DIM];
     for (int d = 0; d < DIM; ++d) {
         fd[d] = F32 ? (double)f32.p[d][i] : f.p[d][i];
-        bd[d] = b.p[d][i];
+        bd[d] = b.p[0] ? b.p[d][i] : 0.0;  // null: the body force is zero
         vd[d] = v.p[d][i];
     }
     for (int d = 0; d < DIM; ++d) {

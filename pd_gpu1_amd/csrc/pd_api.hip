@@ -451,6 +451,9 @@ int force_launch(pd_handle *h, const int *stop)
 
 bool use32(const pd_handle *h) { return h->P.precision == PD_PREC_MIXED; }
 
+// The body force for the Verlet passes: null while it is zero everywhere.
+CD3 body(const pd_handle *h) { return h->b_zero ? CD3{} : cd3(h->b); }
+
 int verlet1_launch(pd_handle *h)
 {
     const pd_params &P = h->P;
@@ -463,7 +466,7 @@ int verlet2_launch(pd_handle *h)
 {
     const pd_params &P = h->P;
     LAUNCH(launch_verlet2(h->stream, P.dim, h->N_int, P.dt, P.dens, h->v, h->a, cd3(h->f), cf3(h->f32), use32(h),
-                          cd3(h->b)));
+                          body(h)));
     return PD_OK;
 }
 
@@ -482,7 +485,7 @@ int fused_step_launch(pd_handle *h)
     if (rc) return rc;
     const pd_params &P = h->P;
     LAUNCH(launch_verlet21(h->stream, P.dim, h->N_int, P.n_top, P.n_bot, P.dt, P.dens, P.strip_velocity, h->v,
-                           cd3(h->f), cf3(h->f32), use32(h), cd3(h->b), h->u, rec_or_null(h)));
+                           cd3(h->f), cf3(h->f32), use32(h), body(h), h->u, rec_or_null(h)));
     return PD_OK;
 }
 
@@ -800,6 +803,7 @@ int pd_set_positions(pd_handle *h, const double *x, const double *y, const doubl
     h->N = n;
     h->N_int = n_int;
     h->xabs = 0.0;
+    h->b_zero = true;  // every field starts at zero
     D3 *sets[] = {&h->x, &h->u, &h->v, &h->a, &h->b, &h->f, &h->vh, &h->vho, &h->fo};
     for (D3 *s : sets) LAUNCH(alloc_set(*s, n, h->stream));
     {
@@ -929,6 +933,7 @@ int pd_init_state(pd_handle *h, int mode)
     LAUNCH(launch_fill_u32(s, h->fail, fail_words(h), 0xFFFFFFFFu));
     D3 *sets[] = {&h->u, &h->v, &h->a, &h->b, &h->f, &h->vh, &h->vho, &h->fo};
     for (D3 *st : sets) LAUNCH(launch_fill_f64(s, st->p[0], 3 * (int64_t)h->N, 0.0));
+    h->b_zero = true;
     LAUNCH(launch_fill_f64(s, h->dmg, h->N, 0.0));
     LAUNCH(hipMemsetAsync(h->f32.p[0], 0, sizeof(float) * 3 * (size_t)h->N, s));
     (void)mode;
@@ -1343,6 +1348,10 @@ int pd_set_body_force_tensile(pd_handle *h, int nx, int ny)
         bx[(size_t)i * nx + nx - 1] = 0.001 * E / dx;
     }
     LAUNCH(copy_on(h->stream, h->b.p[0], bx.data(), sizeof(double) * h->N, hipMemcpyHostToDevice));
+    if (h->b_zero) {
+        h->b_zero = false;
+        drop_graph(h);  // captured Verlet steps skip a zero body force
+    }
     return PD_OK;
 }
 
@@ -1511,6 +1520,10 @@ int pd_set_field(pd_handle *h, int field, const double *in)
     REQUIRE(p != nullptr, "field not allocated");
     REQUIRE(field != PD_FIELD_X0, "reference coordinates are set by pd_set_positions");
     if (field == PD_FIELD_U) h->rec_u_valid = false;
+    if (field == PD_FIELD_B && h->b_zero) {
+        h->b_zero = false;
+        drop_graph(h);  // captured Verlet steps skip a zero body force
+    }
     LAUNCH(hipStreamSynchronize(h->stream));
     if (vec) {
         for (int d = 0; d < h->P.dim; d++)

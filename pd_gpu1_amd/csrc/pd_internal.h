@@ -174,6 +174,7 @@ struct pd_handle {
     double rec_q = 0.0;           // quantum of the reference positions in the records
     double lmin = 0.0;            // shortest reference bond length
     double xabs = 0.0;            // largest |coordinate| of the reference positions
+    bool b_zero = true;           // the body force b is zero everywhere (the Verlet passes skip it)
     double *terms = nullptr;      // [4][N] per-particle terms of the ADR and contact reductions
     int32_t *order = nullptr;     // mixed mode: tiled processing order (null = index order)
     int tile_cells = -1;          // tile edge in units of dx (0 = index order, -1 = default)

@@ -138,6 +138,31 @@ def test_c0_verlet_run_bit_exact_checkpoints():
         assert (m.fail == 0).sum() > (m0.fail == 0).sum()
 
 
+def test_body_force_set_between_verlet_runs_bit_exact():
+    # The Verlet passes skip the body force while it is zero everywhere; a
+    # body force set after steps ran from a captured graph (which skipped
+    # it) must take effect in the next run, as it does in the oracle.
+    sc = scenarios.c0_plate_pmb(scale=0.6)
+    m = _oracle_for(sc)
+    m.init_state()
+    m.set_crack_2d(0.01 * 0.6, 0.0, 0.0, 0.0)  # the scenario's crack at this scale
+    rng = np.random.default_rng(23)
+    b = rng.uniform(-1e9, 1e9, size=(2, len(sc.x)))
+    with sc.make() as prob:
+        prob.run_verlet(12)
+        for _ in range(12):
+            m.step()
+        assert np.array_equal(prob.get("U"), m.stack("U"))
+        prob.set("B", b)
+        m.B = [a.copy() for a in b]
+        prob.run_verlet(12)
+        for _ in range(12):
+            m.step()
+        assert np.array_equal(prob.get("U"), m.stack("U"))
+        assert np.array_equal(prob.get("V"), m.stack("V"))
+        assert np.array_equal(prob.fail_mask(), m.fail)
+
+
 def test_kw_run_with_contact_bit_exact():
     sc = scenarios.c1_kalthoff_winkler(steps=80, width=0.005)
     m = _oracle_for(sc)
