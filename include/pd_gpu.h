@@ -85,7 +85,8 @@ typedef struct pd_params {
     double emod, pratio, dens, s0, dt;
     double break_y_min;        /* 3D breaks need y_i, y_j >= this (Force_Bond.cu:117: -0.04) */
     double strip_velocity;     /* strip load speed (Solve_Plate_Crack_Grow.cu:128: 20 m/s) */
-    double near_window;        /* mixed mode: |s-s0| <= near_window*s0 is re-evaluated in FP64 */
+    double near_window;        /* mixed mode: every bond with |s-s0| <= near_window*s0, and any whose
+                                  FP32 stretch could err across s0, is re-evaluated in FP64 */
 } pd_params;
 
 /* Cylinder impactor (Cylinder_Parameter, Base.cuh:227-236). */
