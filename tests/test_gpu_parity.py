@@ -304,6 +304,29 @@ def test_mixed_pmb3d_force_and_breaks():
         assert (m.fail == 0).sum() > 0
 
 
+def test_mixed_breaks_exact_under_large_translation():
+    # The mixed records hold u rounded to FP32, whose error grows with |u|:
+    # the FP64 re-evaluation band widens with the bond's displacement, so a
+    # large rigid translation on top of a near-critical stretch still breaks
+    # exactly the bonds the FP64 oracle breaks.  Forces lose accuracy with
+    # |u| / |eta| (here ~100), so they are only bounded loosely.
+    sc = scenarios.pmb_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
+    m = _oracle_for(sc)
+    m.init_state()
+    rng = np.random.default_rng(19)
+    u = np.stack([2.05e-3 * sc.x + 0.7, -4e-4 * sc.y - 0.3, 1e-3 * sc.z + 0.45])
+    u = u + rng.uniform(-2e-9, 2e-9, u.shape)
+    with sc.make() as prob:
+        prob.set("U", u)
+        m.U = [a.copy() for a in u]
+        prob.force()
+        m.force()
+        assert np.array_equal(prob.fail_mask(), m.fail)
+        assert (m.fail == 0).sum() > 0
+        f_g, f_o = prob.get("F"), m.stack("F")
+        assert np.max(np.abs(f_g - f_o)) / np.max(np.abs(f_o)) < 5e-2
+
+
 def test_mixed_lps3d_force():
     sc = scenarios.c2_lps_block(nx=30, ny=26, nz=20, precision=_lib.PD_PREC_MIXED)
     m = _oracle_for(sc)
