@@ -3,6 +3,7 @@
 // k_force_lds.hip): particle records, FP32 bond geometry, the FP64 parity
 // stretch, masked bond slots and the break decision.
 #pragma once
+#include "lanes.h"
 #include "pd_internal.h"
 
 namespace {
@@ -62,8 +63,14 @@ __device__ __forceinline__ void bond_break(uint32_t *fail, int32_t k)
 template <int G>
 __device__ __forceinline__ float group_sum(double a, double scale = 1.0)
 {
+    // Butterfly: every lane adds the same pairs in the same order and ends
+    // with the same sum.  Partners within 16 lanes move over DPP.
 #pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    for (int off = G / 2; off > 8; off >>= 1) a += __shfl_xor(a, off, 64);
+    if constexpr (G >= 16) a += lanes::xor_lane<8>(a);
+    if constexpr (G >= 8) a += lanes::xor_lane<4>(a);
+    if constexpr (G >= 4) a += lanes::xor_lane<2>(a);
+    if constexpr (G >= 2) a += lanes::xor_lane<1>(a);
     return (float)(a * scale);
 }
 
