@@ -115,7 +115,6 @@ def test_batch_retire_skips_only_unconverged_specimens():
 
 def test_batch_group_barrier_timeout_is_reported():
     # a one-tick bound on the group barriers' wait: every workgroup leaves the
-    # kernel at its next wait and the increment reports the timeout
     # kernel at its next wait and the increment reports the timeout; the
     # specimens are restored to their state before the increment, so the
     # same handle then runs the increment as a fresh batch does
