@@ -54,10 +54,10 @@ def kernel_stream_bytes(nb, n, dim=3, lds=True, recomputed=True):
     16-bit local index word, which carries the fail bit (2 B; the gathering
     kernels read col 4 B and the fail bit 1/8 B), plus the 8-byte
     coefficient pair when it is streamed rather than recomputed; per particle
-    its own 64-byte record, row_ptr (4 B), the output (4D B) and the staged
+    its own 32-byte record, row_ptr (4 B), the output (4D B) and the staged
     records (counted in the measured traffic, not here)."""
     per_bond = (2.0 if lds else 4.125) + (0.0 if recomputed else 8.0)
-    return per_bond * nb + (64 + 4 + 4 * dim) * n
+    return per_bond * nb + (32 + 4 + 4 * dim) * n
 
 
 def batch_iteration_bytes(nb, n):
@@ -634,7 +634,7 @@ def main():
         # 256 CUs x kernel cycles: 0.97 force, 0.96 dilatation); the
         # dilatation pass's frac passes 1 because its algorithmic count is
         # what a streaming pass would read
-        "limiter": "valu (rocprof: VALU issue 0.97 of cycles; HBM traffic below the algorithmic bytes)",
+        "limiter": "valu (rocprof: VALU issue 0.99 of cycles; HBM traffic below the algorithmic bytes)",
         "measured_frac": (round(traffic / (ms_force * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
         "bytes_per_launch": b_force,
         "ms_per_launch": ms_force,
